@@ -1,0 +1,164 @@
+"""Benchmark: Mcell-RK3-steps/s of the explicit RK3 step on one MI355X.
+
+Workload: viscous Taylor-Green vortex, Re=1600, M=0.1, Pr=0.72, Sutherland
+viscosity, on [0,2 pi]^3 periodic, 256^3 nodes on a 5% sinusoidally perturbed
+curvilinear mesh (seeded phases).  One step = global CFL dt reduction + three
+SSP-RK3 stages (face fluxes, viscous terms, update, ghost fill), inputs
+resident in HBM.  Prints one JSON line.
+
+The roofline object reports the kernel with the largest device time in the
+timed region (HIP events on the block's stream); cpu_baseline times the CPU
+oracle (C + OpenMP restatement of the reference) on a bounded sample of the
+same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+
+# Algorithmic work per unit.  Face flux: FP64 operations of one face of the
+# reference's method (Roe basis + 2 x characteristic WENO5 + LF flux),
+# counted on the oracle with an operation-counting scalar (oracle/opcount.c);
+# until that count is available the hand count 1610 is used.
+FLOPS_PER_FACE = 1610.0
+# HBM bytes per node of one sweep: Q (5) and the sweep's metric (3 + J) read,
+# rhs written (eta) or rhs read + Q_out written (+ Q_n for stages 2-3, zeta).
+BYTES_PER_NODE = {"xfaces": (5 + 4 + 5) * 8, "march_eta": (5 + 4 + 5 + 5) * 8,
+                  "march_zeta": (5 + 4 + 5 + 5 + 5 + 4 + 1) * 8}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(N=128, steps=4):
+    """The oracle on the host cores (bounded sample: TGV N^3, `steps` steps)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle import pyoracle as po
+    from helpers import oracle_block
+    from cbns3d_cuda_amd import cases
+
+    case = cases.taylor_green(N)
+    o = oracle_block(po, case)
+    o.bc_primitive()
+    t = 0.0
+    t, _ = o.rk3_step(1e9, t)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        t, _ = o.rk3_step(1e9, t)
+    el = time.perf_counter() - t0
+    return {"value": N ** 3 * steps / el / 1e6, "unit": "Mcell-RK3-steps/s",
+            "cores": po.lib().orc_threads(), "kind": "port",
+            "sample": "CPU oracle (C+OpenMP), viscous TGV %d^3, %d steps after 1 warm-up, %.1f s"
+                      % (N, steps, el)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-size", type=int, default=128)
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; multi-GPU halo exchange is a design "
+              "(DESIGN.md), not a measured path. Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    from cbns3d_cuda_amd import cases
+    from cbns3d_cuda_amd.block import BUF_Q, F_Q, Block
+
+    N = args.size
+    log("bench: building taylor_green %d^3" % N)
+    case = cases.taylor_green(N)
+    b = Block(case.params)
+    b.calc_metrics(case.x, case.y, case.z, case.period)
+    b.set_conservative(cases.q_ref_aos(case))
+    b.set_bc_calc_primitive(BUF_Q)
+    del case
+
+    t = 0.0
+    for _ in range(args.warmup):
+        t, dt = b.rk3_step(1e9, t)
+    b.synchronize()
+    b.kernel_times(reset=True)
+    b.set_profiling(True)
+    b.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        t, dt = b.rk3_step(1e9, t)
+    b.synchronize()
+    el = time.perf_counter() - t0
+    b.set_profiling(False)
+    kt = b.kernel_times(reset=True)
+    q0 = b.get_field(F_Q)
+    finite = bool(np.isfinite(q0).all())
+    b.free()
+
+    value = N ** 3 * args.steps / el / 1e6
+    log("bench: %.3f ms/step, %.1f Mcell-RK3-steps/s, finite=%s" % (1e3 * el / args.steps, value, finite))
+    for k, (ms, n) in kt.items():
+        if n:
+            log("  %-11s %8.3f ms total  %5d launches  %.4f ms avg" % (k, ms, n, ms / n))
+
+    # dominant kernel among the flux sweeps and its algorithmic work per launch
+    dom = max(kt, key=lambda k: kt[k][0])
+    ms, n = kt[dom]
+    avg_s = ms / n / 1e3
+    nodes = N ** 3
+    if dom in ("xfaces", "march_eta", "march_zeta"):
+        faces = N * N * (N + 1)
+        flops = faces * FLOPS_PER_FACE
+        achieved = flops / avg_s / 1e12
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
+                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
+                "kernel": dom, "avg_ms": ms / n,
+                "note": "FP64 VALU-bound face-flux sweep; peak = MI355X FP64 (vector = matrix)"}
+    else:
+        byts = nodes * 8 * 40
+        achieved = byts / avg_s / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": dom, "avg_ms": ms / n}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        log("bench: cpu baseline (oracle)")
+        cpu = cpu_baseline(args.cpu_size, args.cpu_steps)
+
+    out = {
+        "metric": "Mcell-RK3-steps/s at 256^3: FP64 plain vs FP32+learned hybrid @ L2 err<=1e-6",
+        "value": value if finite else 0.0,
+        "unit": "Mcell-RK3-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * el / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic Taylor-Green initial state, seeded mesh phases)",
+        "config": {"workload": "viscous Taylor-Green vortex %d^3, Re=1600, M=0.1, curvilinear "
+                               "periodic, plain FP64 form" % N, "nodes": nodes,
+                   "form": "fp64-plain", "parallelism": "single GPU"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in kt.items() if v[1]},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,130 @@
+"""Meshes and initial states of the workloads the solver is measured on.
+
+Each builder returns a Case: the cbns_params, node coordinate arrays
+(KM, JM, IM), the period vectors of the periodic axes and the primitive
+initial state at the nodes.  Nothing here runs per step.
+
+  isentropic_vortex  inviscid vortex advection in [0,10]^3, periodic,
+                     curvilinear mesh x = xi + 0.3 sin(2 pi eta/10) sin(2 pi zeta/10)
+                     and cyclic permutations; vortex strength 5 along z
+                     centred at (5,5), freestream u=v=w=1, rho=p=1
+  taylor_green       viscous Taylor-Green vortex on [0,2 pi]^3, periodic,
+                     Re=1600, M=0.1, Pr=0.72, Sutherland viscosity; mesh
+                     x = xi + 0.05 L sin(eta + phi1) sin(zeta + phi2) and
+                     cyclic permutations, phases drawn from a seeded PCG64
+  c_grid_channel     a small mesh for the reference's own boundary set
+                     (far field on xi, wake cut / wall on eta = 0, frozen far
+                     field on eta = JM-1, copies on zeta) with a perturbed
+                     freestream
+"""
+from dataclasses import dataclass
+
+import numpy as np
+
+from .block import BC_PERIODIC, BC_REF, make_params
+
+
+@dataclass
+class Case:
+    name: str
+    params: object
+    x: np.ndarray
+    y: np.ndarray
+    z: np.ndarray
+    period: np.ndarray  # (3, 3) shift of coordinate c across axis a, or None
+    prim: np.ndarray    # (5, KM, JM, IM): rho, u, v, w, p at the nodes
+
+
+def _node_coords(n, length):
+    return np.arange(n, dtype=np.float64) * (length / n)
+
+
+def isentropic_vortex(N=64, *, CFL=0.5, strength=5.0, gamma=1.4):
+    L = 10.0
+    s = _node_coords(N, L)
+    zeta, eta, xi = np.meshgrid(s, s, s, indexing="ij")  # (K, J, I)
+    A, w = 0.3, 2.0 * np.pi / L
+    x = xi + A * np.sin(w * eta) * np.sin(w * zeta)
+    y = eta + A * np.sin(w * zeta) * np.sin(w * xi)
+    z = zeta + A * np.sin(w * xi) * np.sin(w * eta)
+    dx, dy = x - 5.0, y - 5.0
+    r2 = dx * dx + dy * dy
+    e = strength / (2.0 * np.pi) * np.exp(0.5 * (1.0 - r2))
+    u = 1.0 - e * dy
+    v = 1.0 + e * dx
+    wv = np.ones_like(u)
+    T = 1.0 - (gamma - 1.0) * strength ** 2 / (8.0 * gamma * np.pi ** 2) * np.exp(1.0 - r2)
+    rho = T ** (1.0 / (gamma - 1.0))
+    p = rho * T
+    # p_inf = 1/(gamma M^2) = 1 for the freestream rho = p = 1
+    params = make_params(N, N, N, viscous=0, bc=(BC_PERIODIC,) * 3, CFL=CFL, gamma=gamma,
+                         Mach=1.0 / np.sqrt(gamma))
+    period = np.eye(3) * L
+    return Case("isentropic_vortex_%d" % N, params, x, y, z, period, np.stack([rho, u, v, wv, p]))
+
+
+def taylor_green(N=128, *, CFL=0.5, Re=1600.0, Mach=0.1, Pr=0.72, gamma=1.4, T_inf=273.15,
+                 seed=2024, perturb=0.05, visc_centered=0):
+    L = 2.0 * np.pi
+    s = _node_coords(N, L)
+    zeta, eta, xi = np.meshgrid(s, s, s, indexing="ij")
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ph = rng.uniform(0.0, 2.0 * np.pi, size=6)
+    A = perturb * L
+    x = xi + A * np.sin(eta + ph[0]) * np.sin(zeta + ph[1])
+    y = eta + A * np.sin(zeta + ph[2]) * np.sin(xi + ph[3])
+    z = zeta + A * np.sin(xi + ph[4]) * np.sin(eta + ph[5])
+    u = np.sin(x) * np.cos(y) * np.cos(z)
+    v = -np.cos(x) * np.sin(y) * np.cos(z)
+    w = np.zeros_like(u)
+    p0 = 1.0 / (gamma * Mach ** 2)
+    p = p0 + (np.cos(2 * x) + np.cos(2 * y)) * (np.cos(2 * z) + 2.0) / 16.0
+    rho = gamma * Mach ** 2 * p  # T = gamma M^2 p / rho = 1
+    params = make_params(N, N, N, viscous=1, visc_centered=visc_centered,
+                         bc=(BC_PERIODIC,) * 3, CFL=CFL, gamma=gamma, Mach=Mach, Re=Re, Pr=Pr,
+                         T_inf=T_inf)
+    period = np.eye(3) * L
+    return Case("taylor_green_%d" % N, params, x, y, z, period, np.stack([rho, u, v, w, p]))
+
+
+def c_grid_channel(IM=24, JM=16, KM=8, *, viscous=0, seed=7, Mach=0.5, CFL=0.5,
+                   angle_attack=2.31):
+    """Curved sheared mesh with the reference's boundary set on all axes."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    k, j, i = np.meshgrid(np.arange(KM, dtype=float), np.arange(JM, dtype=float),
+                          np.arange(IM, dtype=float), indexing="ij")
+    x = i * 0.1 + 0.02 * np.sin(0.3 * j) * np.cos(0.2 * k)
+    y = j * 0.08 * (1.0 + 0.05 * np.cos(0.25 * i)) + 0.01 * i
+    z = k * 0.12 + 0.01 * np.sin(0.2 * i + 0.1 * j)
+    gamma = 1.4
+    aoa = angle_attack * np.pi / 180.0
+    shape = x.shape
+    rho = 1.0 + 0.05 * rng.standard_normal(shape)
+    u = np.cos(aoa) + 0.05 * rng.standard_normal(shape)
+    v = np.sin(aoa) + 0.05 * rng.standard_normal(shape)
+    w = 0.05 * rng.standard_normal(shape)
+    p = 1.0 / (gamma * Mach ** 2) * (1.0 + 0.05 * rng.standard_normal(shape))
+    params = make_params(IM, JM, KM, viscous=viscous, bc=(BC_REF,) * 3, CFL=CFL, gamma=gamma,
+                         Mach=Mach, Re=5.0e3, Pr=0.71, T_inf=255.556, angle_attack=angle_attack,
+                         i_begin=max(2, IM // 4), i_end=IM - max(2, IM // 4) + 1)
+    return Case("c_grid_channel", params, x, y, z, None, np.stack([rho, u, v, w, p]))
+
+
+def conservative(prim, gamma):
+    """(5, ...) primitives -> (5, ...) conservative (src/conservative_kernel.cu:42-46)."""
+    rho, u, v, w, p = prim
+    return np.stack([rho, rho * u, rho * v, rho * w,
+                     p / (gamma - 1.0) + 0.5 * rho * (u * u + v * v + w * w)])
+
+
+def ghosted(arr, ng, mode="edge"):
+    """Pad node arrays (..., K, J, I) with ng ghost layers on each side."""
+    pad = [(0, 0)] * (arr.ndim - 3) + [(ng, ng)] * 3
+    return np.pad(arr, pad, mode=mode)
+
+
+def q_ref_aos(case):
+    """Conservative state in the reference's host layout: AoS, 2 ghost layers."""
+    q = conservative(case.prim, case.params.gamma)
+    qg = ghosted(q, 2, mode="wrap" if case.period is not None else "edge")
+    return np.ascontiguousarray(np.moveaxis(qg, 0, -1))

@@ -1,0 +1,194 @@
+// k_fused.hip -- one RK3 stage as sweep kernels (cbns_rk3_stage):
+//
+//   k_xfaces     one thread per xi-face, lanes along i (coalesced); the face
+//                flux goes to HBM (5 doubles per face)
+//   k_march<1>   lanes along i, each thread marches a chunk of nodes along
+//                eta holding the 6-node stencil and the previous face flux in
+//                registers: each node is loaded once per sweep and each face
+//                is computed once.  Writes rhs = dE + dF.
+//   [viscous]    dfv from the viscous kernels
+//   k_march<2>   the same march along zeta; adds dG, subtracts the viscous
+//                divergence, divides by J and applies the Shu-Osher stage
+//                combination, writing the stage output.
+//
+// Node primitives for the Roe average are recomputed from Q (the
+// primitive array holds exactly f(Q) at every node a face reads:
+// boundary rules write Q first and derive the primitives from it).
+#include "cbns_internal.hpp"
+#include "face_math.hpp"
+
+namespace cbns {
+
+namespace {
+
+constexpr int MARCH_X = 64;  // lanes along i
+constexpr int MARCH_Y = 4;   // lines per workgroup along the other axis
+constexpr int CHUNK = 32;    // nodes per thread along the march axis
+
+template <class T>
+__device__ __forceinline__ void load_q(const double* __restrict__ Q, long S, int o, T q[5]) {
+#pragma unroll
+  for (int n = 0; n < 5; n++) q[n] = (T)Q[n * S + o];
+}
+
+__global__ void __launch_bounds__(256) k_xfaces(Consts c, Box box, const double* __restrict__ Q,
+                                                const double* __restrict__ met,
+                                                double* __restrict__ fx) {
+  int i, j, k;
+  if (!box_node(box, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  const long S = c.S;
+  const int a = idx(c, i, j, k), b = a + 1;
+  const double Ja = met[JAC * S + a], Jb = met[JAC * S + b];
+  const double nx = 0.5 * (met[XI_X * S + a] * Ja + met[XI_X * S + b] * Jb);
+  const double ny = 0.5 * (met[XI_Y * S + a] * Ja + met[XI_Y * S + b] * Jb);
+  const double nz = 0.5 * (met[XI_Z * S + a] * Ja + met[XI_Z * S + b] * Jb);
+  double q[6][5], pl[5], pr[5], f[5];
+#pragma unroll
+  for (int s = 0; s < 6; s++) load_q(Q, S, a + (s - 2), q[s]);
+  const GasT<double> g{c.gamma, c.gam1, c.gam1_inv};
+  q_to_prim(g, q[2], pl[0], pl[1], pl[2], pl[3], pl[4]);
+  q_to_prim(g, q[3], pr[0], pr[1], pr[2], pr[3], pr[4]);
+  face_flux<double>(g, q, pl, pr, nx, ny, nz, f);
+#pragma unroll
+  for (int n = 0; n < 5; n++) fx[n * S + a] = f[n];
+}
+
+struct StageArgs {
+  double dt;
+  int stage;
+  const double* Qn;     // level-n state (stages 2, 3)
+  const double* Qin;    // stage input (stencil source)
+  double* Qout;         // stage output
+  const double* fx;     // xi-face fluxes (AX == 1)
+  double* rhs;          // dE + dF (+ viscous)
+  const double* dfv;    // viscous divergence or nullptr
+};
+
+// AX = 1: march along eta, rhs = dE + dF.  AX = 2: march along zeta, finish.
+template <int AX>
+__global__ void __launch_bounds__(MARCH_X * MARCH_Y) k_march(Consts c, StageArgs sa,
+                                                            const double* __restrict__ met) {
+  const int ax_t = AX == 1 ? 2 : 1;  // transverse axis other than i
+  const int i = upd_lo(c, 0) + blockIdx.x * MARCH_X + threadIdx.x;
+  const int t = upd_lo(c, ax_t) + blockIdx.y * MARCH_Y + threadIdx.y;
+  if (i > upd_hi(c, 0) || t > upd_hi(c, ax_t)) return;
+  const int mlo = upd_lo(c, AX), mhi = upd_hi(c, AX);
+  const int m0 = mlo + blockIdx.z * CHUNK;
+  if (m0 > mhi) return;
+  const int m1 = min(m0 + CHUNK, mhi + 1);
+  const long S = c.S;
+  const int st = AX == 1 ? c.PX : c.PXY;
+  const int base = AX == 1 ? idx(c, i, 0, t) : idx(c, i, t, 0);  // node m at base + m*st
+  const double* __restrict__ Q = sa.Qin;
+  const double* __restrict__ nX = met + (3 * AX + 0) * S;
+  const double* __restrict__ nY = met + (3 * AX + 1) * S;
+  const double* __restrict__ nZ = met + (3 * AX + 2) * S;
+  const double* __restrict__ J = met + JAC * S;
+  const GasT<double> g{c.gamma, c.gam1, c.gam1_inv};
+
+  double q[6][5];  // nodes m-2 .. m+3 for the face m+1/2
+#pragma unroll
+  for (int s = 0; s < 6; s++) load_q(Q, S, base + (m0 - 3 + s) * st, q[s]);
+  // J grad xi_AX at the face's two nodes
+  int o = base + (m0 - 1) * st;
+  double ja = J[o], jb = J[o + st];
+  double ax = nX[o] * ja, ay = nY[o] * ja, az = nZ[o] * ja;
+  double bx = nX[o + st] * jb, by = nY[o + st] * jb, bz = nZ[o + st] * jb;
+  double fprev[5];
+  {
+    double pl[5], pr[5];
+    q_to_prim(g, q[2], pl[0], pl[1], pl[2], pl[3], pl[4]);
+    q_to_prim(g, q[3], pr[0], pr[1], pr[2], pr[3], pr[4]);
+    face_flux<double>(g, q, pl, pr, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), fprev);
+  }
+  for (int m = m0; m < m1; m++) {
+    // shift the window by one node
+#pragma unroll
+    for (int s = 0; s < 5; s++)
+#pragma unroll
+      for (int n = 0; n < 5; n++) q[s][n] = q[s + 1][n];
+    o = base + m * st;
+    load_q(Q, S, o + 3 * st, q[5]);
+    ax = bx; ay = by; az = bz;
+    jb = J[o + st];
+    bx = nX[o + st] * jb; by = nY[o + st] * jb; bz = nZ[o + st] * jb;
+    double f[5];
+    {
+      double pl[5], pr[5];
+      q_to_prim(g, q[2], pl[0], pl[1], pl[2], pl[3], pl[4]);
+      q_to_prim(g, q[3], pr[0], pr[1], pr[2], pr[3], pr[4]);
+      face_flux<double>(g, q, pl, pr, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), f);
+    }
+    if (AX == 1) {
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        const double* fxn = sa.fx + n * S;
+        sa.rhs[n * S + o] = (fxn[o] - fxn[o - 1]) + (f[n] - fprev[n]);
+      }
+    } else {
+      const double inv_j = 1.0 / J[o];
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        double df = sa.rhs[n * S + o] + (f[n] - fprev[n]);
+        if (n > 0 && sa.dfv) df -= sa.dfv[(n - 1) * S + o];
+        df *= inv_j;
+        const double qin = q[2][n];
+        double out;
+        if (sa.stage == 1) out = qin - sa.dt * df;
+        else if (sa.stage == 2) out = 0.75 * sa.Qn[n * S + o] + 0.25 * (qin - sa.dt * df);
+        else out = (sa.Qn[n * S + o] + 2.0 * (qin - sa.dt * df)) * (1.0 / 3.0);
+        sa.Qout[n * S + o] = out;
+      }
+    }
+#pragma unroll
+    for (int n = 0; n < 5; n++) fprev[n] = f[n];
+  }
+}
+
+}  // namespace
+
+int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout) {
+  const Consts& c = b->c;
+  double* fx = b->flux;  // first 5 face arrays
+  {
+    Box box;
+    for (int a = 0; a < 3; a++) {
+      box.lo[a] = upd_lo(c, a);
+      box.n[a] = upd_hi(c, a) - upd_lo(c, a) + 1;
+    }
+    box.lo[0] = periodic(c, 0) ? -1 : 0;
+    box.n[0] = c.IM - 1 + (periodic(c, 0) ? 1 : 0);
+    ProfScope ps(b, CBNS_K_XFACES);
+    k_xfaces<<<box_blocks(box, 256), 256, 0, b->stream>>>(c, box, qin, b->met, fx);
+    CBNS_HIP(b, hipGetLastError());
+  }
+  StageArgs sa{dt, stage, b->Q, qin, qout, fx, b->rhs, nullptr};
+  const int ni = upd_hi(c, 0) - upd_lo(c, 0) + 1;
+  {
+    const int nt = upd_hi(c, 2) - upd_lo(c, 2) + 1;
+    const int nm = upd_hi(c, 1) - upd_lo(c, 1) + 1;
+    dim3 grid((ni + MARCH_X - 1) / MARCH_X, (nt + MARCH_Y - 1) / MARCH_Y,
+              (nm + CHUNK - 1) / CHUNK);
+    ProfScope ps(b, CBNS_K_MARCH_ETA);
+    k_march<1><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
+    CBNS_HIP(b, hipGetLastError());
+  }
+  if (c.viscous) {
+    ProfScope ps(b, CBNS_K_VISCOUS);
+    int rc = launch_viscous_unfused(b);
+    if (rc) return rc;
+    sa.dfv = b->dfv;
+  }
+  {
+    const int nt = upd_hi(c, 1) - upd_lo(c, 1) + 1;
+    const int nm = upd_hi(c, 2) - upd_lo(c, 2) + 1;
+    dim3 grid((ni + MARCH_X - 1) / MARCH_X, (nt + MARCH_Y - 1) / MARCH_Y,
+              (nm + CHUNK - 1) / CHUNK);
+    ProfScope ps(b, CBNS_K_MARCH_ZETA);
+    k_march<2><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
+    CBNS_HIP(b, hipGetLastError());
+  }
+  return CBNS_OK;
+}
+
+}  // namespace cbns

@@ -1,0 +1,875 @@
+/*
+ * cbns_oracle.c -- CPU restatement (plain C + OpenMP) of the reference's RK3
+ * step.  TEST INFRASTRUCTURE ONLY: the checker for the HIP library and the
+ * timed CPU baseline.  The product never links it.
+ *
+ * Operation order follows the reference line by line; citations are
+ * path:line in chenbei102/cbns3d_cuda.  Built with -ffp-contract=off so every
+ * + - * / sqrt is a single IEEE operation (the reference's nvcc build
+ * contracts FMAs where it chooses, so bit-equality with it is not defined;
+ * parity is the 1e-12 relative bar per RK stage).
+ *
+ * Parity pins (tests/test_oracle_*.py):
+ *   orc_weno5            constant stencil -> constant (1/6 for 4,1,0,1,4),
+ *                        (-8,-1,0,1,8) -> 0.49954946749288827 (mpmath value),
+ *                        5th-order linear weights on smooth polynomials,
+ *                        step -> smooth sub-stencil, mpmath on random stencils
+ *   orc_eigenvectors_roe L.R = I, R columns are eigenvectors of the Euler flux
+ *                        Jacobian A(n) with eigenvalues qn-c, qn, qn+c, qn, qn,
+ *                        both tangent branches, equal states -> that state
+ *   orc_rec_weno5        R.weno(L.Q) on a constant stencil returns the state
+ *   orc_lf_flux          equal states -> F(Q).n; n = x -> classic Rusanov
+ *   orc_face_flux        freestream -> F(Q).n; brute-force numpy restatement
+ *   orc_gradient_nodes   the reference's own gradient gtests (i^2+j^2+k^2,
+ *                        ijk), exact for degree <= 4
+ *   orc_calc_metrics     the reference's three metrics gtests (tanh-stretched,
+ *                        rotated, conical shell); periodic: closed form of the
+ *                        4th-order stencil on a sine
+ *   orc_bc_primitive     periodic: brute-force wrap restatement; ref: numpy
+ *                        restatement of each phase on small grids
+ *   orc_numerical_flux   uniform flow -> exact zero divergence on uniform mesh
+ *   orc_viscous          linear velocity/temperature fields on a Cartesian
+ *                        mesh -> closed-form tau, q, zero divergence; the
+ *                        transverse offset shows on transverse-varying fields
+ *   orc_update_rk3       Shu-Osher coefficients on synthetic df
+ *   orc_calc_dt          uniform Cartesian mesh + uniform flow -> closed form
+ */
+#include "cbns_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define NEQ 5
+#define G ORC_G
+
+int orc_threads(void) {
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+/* src/Block3d.cpp:99-113 (derived constants) */
+int orc_grid_init(orc_grid* g, const orc_params* p) {
+  if (p->IM < 5 || p->JM < 5 || p->KM < 5) return -1;
+  memset(g, 0, sizeof(*g));
+  g->IM = p->IM; g->JM = p->JM; g->KM = p->KM;
+  g->PX = p->IM + 2 * G; g->PY = p->JM + 2 * G; g->PZ = p->KM + 2 * G;
+  g->S = g->PX * g->PY * g->PZ;
+  g->i_begin = p->i_begin; g->i_end = p->i_end;
+  g->viscous = p->viscous; g->visc_centered = p->visc_centered;
+  for (int a = 0; a < 3; a++) g->bc[a] = p->bc[a];
+  g->CFL = p->CFL; g->gamma = p->gamma; g->Mach = p->Mach; g->Re = p->Re;
+  g->Pr = p->Pr; g->Pr_t = p->Pr_t; g->T_inf = p->T_inf;
+  g->angle_attack = p->angle_attack;
+  g->Mach2 = g->Mach * g->Mach;
+  g->Re_inv = 1.0 / g->Re;
+  g->Pr_inv = 1.0 / g->Pr;
+  g->Pr_t_inv = 1.0 / g->Pr_t;
+  g->gam1 = g->gamma - 1.0;
+  g->gM2 = g->gamma * g->Mach2;
+  g->gam1_inv = 1.0 / g->gam1;
+  g->gPr = g->gamma / g->Pr;
+  g->p_inf = 1.0 / g->gM2;
+  g->C_T_inf = 110.5 / g->T_inf;
+  {
+    double a = 4.0 / 3.0, b = g->gamma * g->Pr_inv;
+    g->C_dt_v = (a > b ? a : b) * g->Re_inv;
+  }
+  return 0;
+}
+
+long orc_idx(const orc_grid* g, int i, int j, int k) {
+  return (long)(i + G) + g->PX * ((long)(j + G) + g->PY * (long)(k + G));
+}
+#define IDX(i, j, k) orc_idx(g, (i), (j), (k))
+
+static int axis_n(const orc_grid* g, int a) { return a == 0 ? g->IM : (a == 1 ? g->JM : g->KM); }
+static int periodic(const orc_grid* g, int a) { return g->bc[a] == ORC_BC_PERIODIC; }
+/* nodes the RK update touches along an axis: src/runge_kutta_kernel.cu:24-34
+ * (interior only) for reference axes, every node for periodic axes */
+static int upd_lo(const orc_grid* g, int a) { return periodic(g, a) ? 0 : 1; }
+static int upd_hi(const orc_grid* g, int a) { return periodic(g, a) ? axis_n(g, a) - 1 : axis_n(g, a) - 2; }
+
+/* ------------------------------------------------------------------------ */
+/* src/weno.cu:13-44 */
+double orc_weno5(double Um2, double Um1, double U0, double Up1, double Up2) {
+  const double eps = 1.0e-6;
+  double diff1 = Um2 - 2.0 * Um1 + U0;
+  double diff2 = Um2 - 4.0 * Um1 + 3.0 * U0;
+  double diff3 = Um1 - 2.0 * U0 + Up1;
+  double diff4 = Um1 - Up1;
+  double diff5 = U0 - 2.0 * Up1 + Up2;
+  double diff6 = 3.0 * U0 - 4.0 * Up1 + Up2;
+  double IS1 = (13.0 / 12.0) * diff1 * diff1 + 0.25 * diff2 * diff2 + eps;
+  double IS2 = (13.0 / 12.0) * diff3 * diff3 + 0.25 * diff4 * diff4 + eps;
+  double IS3 = (13.0 / 12.0) * diff5 * diff5 + 0.25 * diff6 * diff6 + eps;
+  double w1 = 0.1 / (IS1 * IS1);
+  double w2 = 0.6 / (IS2 * IS2);
+  double w3 = 0.3 / (IS3 * IS3);
+  double sum = w1 + w2 + w3;
+  double f1 = (2.0 * Um2 - 7.0 * Um1 + 11.0 * U0) / 6.0;
+  double f2 = (-Um1 + 5.0 * U0 + 2.0 * Up1) / 6.0;
+  double f3 = (2.0 * U0 + 5.0 * Up1 - Up2) / 6.0;
+  return (w1 * f1 + w2 * f2 + w3 * f3) / sum;
+}
+
+/* src/calc_eigenvectors_roe.cu:13-152.  pl/pr = (rho,u,v,w,p); R, L row-major */
+void orc_eigenvectors_roe(const orc_grid* g, const double pl[5], const double pr[5],
+                          double n_x, double n_y, double n_z, double R[25], double L[25]) {
+  const double gamma = g->gamma, gam1 = g->gam1;
+  const double rho_L = pl[0], u_L = pl[1], v_L = pl[2], w_L = pl[3], p_L = pl[4];
+  const double rho_R = pr[0], u_R = pr[1], v_R = pr[2], w_R = pr[3], p_R = pr[4];
+  double l_x = 0.0, l_y = n_z, l_z = -n_y;
+  double m_x, m_y, m_z;
+  double n_abs = sqrt(l_x * l_x + l_y * l_y + l_z * l_z);
+  if (n_abs > 1.0e-8) {
+    m_x = -n_y * n_y - n_z * n_z;
+    m_y = n_x * n_y;
+    m_z = n_x * n_z;
+  } else {
+    l_x = -n_z; l_y = 0.0; l_z = n_x;
+    m_x = n_x * n_y;
+    m_y = -n_x * n_x - n_z * n_z;
+    m_z = n_y * n_z;
+  }
+  n_abs = sqrt(n_x * n_x + n_y * n_y + n_z * n_z);
+  n_x /= n_abs; n_y /= n_abs; n_z /= n_abs;
+  n_abs = sqrt(l_x * l_x + l_y * l_y + l_z * l_z);
+  l_x /= n_abs; l_y /= n_abs; l_z /= n_abs;
+  n_abs = sqrt(m_x * m_x + m_y * m_y + m_z * m_z);
+  m_x /= n_abs; m_y /= n_abs; m_z /= n_abs;
+
+  double c_L = sqrt(gamma * p_L / rho_L);
+  double H_L = c_L * c_L / gam1 + 0.5 * (u_L * u_L + v_L * v_L + w_L * w_L);
+  double c_R = sqrt(gamma * p_R / rho_R);
+  double H_R = c_R * c_R / gam1 + 0.5 * (u_R * u_R + v_R * v_R + w_R * w_R);
+  double ratio = sqrt(rho_R / rho_L);
+  double uu = (u_L + ratio * u_R) / (1.0 + ratio);
+  double vv = (v_L + ratio * v_R) / (1.0 + ratio);
+  double ww = (w_L + ratio * w_R) / (1.0 + ratio);
+  double H = (H_L + ratio * H_R) / (1.0 + ratio);
+  double q2 = uu * uu + vv * vv + ww * ww;
+  double c = sqrt(gam1 * (H - 0.5 * q2));
+  double qn = uu * n_x + vv * n_y + ww * n_z;
+  double ql = uu * l_x + vv * l_y + ww * l_z;
+  double qm = uu * m_x + vv * m_y + ww * m_z;
+
+#define RR(r, cc) R[(r) * 5 + (cc)]
+#define LL(r, cc) L[(r) * 5 + (cc)]
+  RR(0, 0) = 1.0; RR(1, 0) = uu - c * n_x; RR(2, 0) = vv - c * n_y; RR(3, 0) = ww - c * n_z; RR(4, 0) = H - qn * c;
+  RR(0, 1) = 1.0; RR(1, 1) = uu;           RR(2, 1) = vv;           RR(3, 1) = ww;           RR(4, 1) = 0.5 * q2;
+  RR(0, 2) = 1.0; RR(1, 2) = uu + c * n_x; RR(2, 2) = vv + c * n_y; RR(3, 2) = ww + c * n_z; RR(4, 2) = H + qn * c;
+  RR(0, 3) = 0.0; RR(1, 3) = l_x;          RR(2, 3) = l_y;          RR(3, 3) = l_z;          RR(4, 3) = ql;
+  RR(0, 4) = 0.0; RR(1, 4) = m_x;          RR(2, 4) = m_y;          RR(3, 4) = m_z;          RR(4, 4) = qm;
+
+  double K_2c2 = gam1 / (2.0 * c * c);
+  double one_2c = 0.5 / c;
+  LL(0, 0) = 0.5 * K_2c2 * q2 + one_2c * qn;
+  LL(1, 0) = 1.0 - K_2c2 * q2;
+  LL(2, 0) = 0.5 * K_2c2 * q2 - one_2c * qn;
+  LL(3, 0) = -ql;
+  LL(4, 0) = -qm;
+  LL(0, 1) = -(K_2c2 * uu + one_2c * n_x);
+  LL(1, 1) = 2.0 * K_2c2 * uu;
+  LL(2, 1) = -(K_2c2 * uu - one_2c * n_x);
+  LL(3, 1) = l_x;
+  LL(4, 1) = m_x;
+  LL(0, 2) = -(K_2c2 * vv + one_2c * n_y);
+  LL(1, 2) = 2.0 * K_2c2 * vv;
+  LL(2, 2) = -(K_2c2 * vv - one_2c * n_y);
+  LL(3, 2) = l_y;
+  LL(4, 2) = m_y;
+  LL(0, 3) = -(K_2c2 * ww + one_2c * n_z);
+  LL(1, 3) = 2.0 * K_2c2 * ww;
+  LL(2, 3) = -(K_2c2 * ww - one_2c * n_z);
+  LL(3, 3) = l_z;
+  LL(4, 3) = m_z;
+  LL(0, 4) = K_2c2;
+  LL(1, 4) = -2.0 * K_2c2;
+  LL(2, 4) = K_2c2;
+  LL(3, 4) = 0.0;
+  LL(4, 4) = 0.0;
+#undef RR
+#undef LL
+}
+
+/* src/weno.cu:46-120.  f = 5 stencil points x 5 conservative components
+ * (f[p*5+c]); output prim = (rho,u,v,w,p) of the reconstructed state. */
+void orc_rec_weno5(const orc_grid* g, const double f[25], const double R[25],
+                   const double L[25], double prim[5]) {
+  double f_sten[NEQ][5], fc_sten[NEQ][5];
+  for (int pnt = 0; pnt < 5; pnt++)
+    for (int c = 0; c < NEQ; c++) f_sten[c][pnt] = f[pnt * 5 + c];
+  for (int li = 0; li < NEQ; li++)
+    for (int lj = 0; lj < 5; lj++) {
+      double ss = 0.0;
+      for (int lk = 0; lk < NEQ; lk++) ss += L[li * 5 + lk] * f_sten[lk][lj];
+      fc_sten[li][lj] = ss;
+    }
+  for (int li = 0; li < NEQ; li++)
+    f_sten[li][0] = orc_weno5(fc_sten[li][0], fc_sten[li][1], fc_sten[li][2],
+                              fc_sten[li][3], fc_sten[li][4]);
+  for (int li = 0; li < NEQ; li++) {
+    double ss = 0.0;
+    for (int lj = 0; lj < NEQ; lj++) ss += R[li * 5 + lj] * f_sten[lj][0];
+    f_sten[li][1] = ss;
+  }
+  double rho = f_sten[0][1];
+  double u = f_sten[1][1] / rho;
+  double v = f_sten[2][1] / rho;
+  double w = f_sten[3][1] / rho;
+  double p = g->gam1 * (f_sten[4][1] - 0.5 * rho * (u * u + v * v + w * w));
+  prim[0] = rho; prim[1] = u; prim[2] = v; prim[3] = w; prim[4] = p;
+}
+
+/* src/lf_flux.cu:13-80 (note the argument order: right state first) */
+void orc_lf_flux(const orc_grid* g, const double pr[5], const double pl[5],
+                 double nx, double ny, double nz, double flux[5]) {
+  const double gamma = g->gamma, gam1 = g->gam1;
+  const double rho_R = pr[0], u_R = pr[1], v_R = pr[2], w_R = pr[3], p_R = pr[4];
+  const double rho_L = pl[0], u_L = pl[1], v_L = pl[2], w_L = pl[3], p_L = pl[4];
+  double c_R = sqrt(gamma * p_R / rho_R);
+  double H_R = c_R * c_R / gam1 + 0.5 * (u_R * u_R + v_R * v_R + w_R * w_R);
+  double rE_R = p_R / gam1 + 0.5 * rho_R * (u_R * u_R + v_R * v_R + w_R * w_R);
+  double c_L = sqrt(gamma * p_L / rho_L);
+  double H_L = c_L * c_L / gam1 + 0.5 * (u_L * u_L + v_L * v_L + w_L * w_L);
+  double rE_L = p_L / gam1 + 0.5 * rho_L * (u_L * u_L + v_L * v_L + w_L * w_L);
+  double q_R = u_R * nx + v_R * ny + w_R * nz;
+  double q_L = u_L * nx + v_L * ny + w_L * nz;
+  double ws[3], ws_max;
+  ws[0] = fabs(u_R) + c_R; ws_max = fabs(u_L) + c_L; if (ws_max > ws[0]) ws[0] = ws_max;
+  ws[1] = fabs(v_R) + c_R; ws_max = fabs(v_L) + c_L; if (ws_max > ws[1]) ws[1] = ws_max;
+  ws[2] = fabs(w_R) + c_R; ws_max = fabs(w_L) + c_L; if (ws_max > ws[2]) ws[2] = ws_max;
+  ws_max = fabs(ws[0] * nx + ws[1] * ny + ws[2] * nz);
+  double diss[NEQ], f_L[NEQ], f_R[NEQ];
+  diss[0] = ws_max * (rho_R - rho_L);
+  diss[1] = ws_max * (rho_R * u_R - rho_L * u_L);
+  diss[2] = ws_max * (rho_R * v_R - rho_L * v_L);
+  diss[3] = ws_max * (rho_R * w_R - rho_L * w_L);
+  diss[4] = ws_max * (rE_R - rE_L);
+  f_L[0] = rho_L * q_L;
+  f_L[1] = rho_L * q_L * u_L + p_L * nx;
+  f_L[2] = rho_L * q_L * v_L + p_L * ny;
+  f_L[3] = rho_L * q_L * w_L + p_L * nz;
+  f_L[4] = rho_L * q_L * H_L;
+  f_R[0] = rho_R * q_R;
+  f_R[1] = rho_R * q_R * u_R + p_R * nx;
+  f_R[2] = rho_R * q_R * v_R + p_R * ny;
+  f_R[3] = rho_R * q_R * w_R + p_R * nz;
+  f_R[4] = rho_R * q_R * H_R;
+  for (int li = 0; li < NEQ; li++) flux[li] = 0.5 * (f_L[li] + f_R[li] - diss[li]);
+}
+
+/* One face of src/rec_riemann_kernel.cu:44-89: q holds the 6 nodes m-2..m+3
+ * along the sweep (q[s*5+c]); pl/pr are the node primitives at m and m+1;
+ * (nx,ny,nz) the averaged unnormalised face vector. */
+void orc_face_flux(const orc_grid* g, const double q[30], const double pl[5],
+                   const double pr[5], double nx, double ny, double nz, double flux[5]) {
+  double R[25], L[25], sl[25], sr[25], wl[5], wr[5];
+  orc_eigenvectors_roe(g, pl, pr, nx, ny, nz, R, L);
+  for (int p = 0; p < 5; p++)
+    for (int c = 0; c < 5; c++) {
+      sl[p * 5 + c] = q[p * 5 + c];             /* m-2 .. m+2 */
+      sr[p * 5 + c] = q[(5 - p) * 5 + c];       /* m+3 .. m-1 */
+    }
+  orc_rec_weno5(g, sl, R, L, wl);
+  orc_rec_weno5(g, sr, R, L, wr);
+  orc_lf_flux(g, wr, wl, nx, ny, nz, flux);
+}
+
+/* ------------------------------------------------------------------------ */
+/* gradient stencils, src/gradient_kernel.cu:40-131 (one axis).  fm(o) returns
+ * f at offset o from the node; p is the node position in [0, n). */
+#define GRAD1(p, n, F0, Fm1, Fm2, Fm3, Fm4, Fp1, Fp2, Fp3, Fp4, out)                      \
+  do {                                                                                   \
+    if (0 == (p)) out = -25.0 / 12.0 * (F0) + 4.0 * (Fp1) - 3.0 * (Fp2) + (4.0 / 3.0) * (Fp3) - 0.25 * (Fp4); \
+    if (1 == (p)) out = -5.0 / 6.0 * (F0) - 0.25 * (Fm1) + 1.5 * (Fp1) - 0.5 * (Fp2) + (1.0 / 12.0) * (Fp3); \
+    if ((1 < (p)) && ((n) - 2 > (p)))                                                   \
+      out = (1.0 / 12.0) * (Fm2) - 2.0 / 3.0 * (Fm1) + (2.0 / 3.0) * (Fp1) - 1.0 / 12.0 * (Fp2); \
+    if ((n) - 2 == (p)) out = (5.0 / 6.0) * (F0) - 1.0 / 12.0 * (Fm3) + 0.5 * (Fm2) - 1.5 * (Fm1) + 0.25 * (Fp1); \
+    if ((n) - 1 == (p)) out = (25.0 / 12.0) * (F0) + 0.25 * (Fm4) - 4.0 / 3.0 * (Fm3) + 3.0 * (Fm2) - 4.0 * (Fm1); \
+  } while (0)
+
+static inline double central4(double fm2, double fm1, double fp1, double fp2) {
+  return (1.0 / 12.0) * fm2 - 2.0 / 3.0 * fm1 + (2.0 / 3.0) * fp1 - 1.0 / 12.0 * fp2;
+}
+
+/* src/gradient_kernel.cu:13-133 on a node array IM x JM x KM (i fastest) */
+void orc_gradient_nodes(int IM, int JM, int KM, const double* f,
+                        double* fx, double* fy, double* fz) {
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < KM; k++)
+    for (int j = 0; j < JM; j++)
+      for (int i = 0; i < IM; i++) {
+        long s[3] = {1, IM, (long)IM * JM};
+        int pos[3] = {i, j, k}, n[3] = {IM, JM, KM};
+        double* out[3] = {fx, fy, fz};
+        long i0 = i + (long)IM * (j + (long)JM * k);
+        for (int a = 0; a < 3; a++) {
+          int p = pos[a];
+#define FV(o) ((p + (o) >= 0 && p + (o) < n[a]) ? f[i0 + (o) * s[a]] : 0.0)
+          double r = 0.0;
+          GRAD1(p, n[a], FV(0), FV(-1), FV(-2), FV(-3), FV(-4), FV(1), FV(2), FV(3), FV(4), r);
+#undef FV
+          out[a][i0] = r;
+        }
+      }
+}
+
+/* map a ghosted coordinate to its source node for metric ghost fill:
+ * clamp for reference axes (src/metrics_kernel.cu:72-560 copies the
+ * nearest boundary node into ghost faces, edges and corners), wrap for
+ * periodic axes */
+static int ghost_src(const orc_grid* g, int a, int m) {
+  int n = axis_n(g, a);
+  if (periodic(g, a)) return ((m % n) + n) % n;
+  return m < 0 ? 0 : (m >= n ? n - 1 : m);
+}
+
+/* src/calc_metrics.cu:16-111, src/metrics_kernel.cu:13-562.  Coordinates are
+ * node arrays IM x JM x KM.  For a periodic axis a the node beyond the end
+ * is the first node shifted by period[3*a + c] in coordinate c, and the
+ * coordinate derivative uses the central stencil everywhere. */
+int orc_calc_metrics(const orc_grid* g, const double* x, const double* y,
+                     const double* z, const double period[9], double* met) {
+  const int IM = g->IM, JM = g->JM, KM = g->KM;
+  const long NN = (long)IM * JM * KM;
+  double* d = (double*)malloc(sizeof(double) * 9 * NN); /* x_xi..z_zeta */
+  if (!d) return -1;
+  const double* X[3] = {x, y, z};
+  int bad = 0;
+#pragma omp parallel for schedule(static) reduction(| : bad)
+  for (int k = 0; k < KM; k++)
+    for (int j = 0; j < JM; j++)
+      for (int i = 0; i < IM; i++) {
+        int pos[3] = {i, j, k}, n[3] = {IM, JM, KM};
+        long s[3] = {1, IM, (long)IM * JM};
+        long i0 = i + (long)IM * (j + (long)JM * k);
+        for (int c = 0; c < 3; c++)
+          for (int a = 0; a < 3; a++) {
+            double r = 0.0;
+            int p = pos[a];
+            if (periodic(g, a)) {
+              double fv[5];
+              for (int o = -2; o <= 2; o++) {
+                int q = p + o, wrap = 0;
+                while (q < 0) { q += n[a]; wrap--; }
+                while (q >= n[a]) { q -= n[a]; wrap++; }
+                fv[o + 2] = X[c][i0 + (long)(q - p) * s[a]] + wrap * period[3 * a + c];
+              }
+              r = central4(fv[0], fv[1], fv[3], fv[4]);
+            } else {
+#define FV(o) ((p + (o) >= 0 && p + (o) < n[a]) ? X[c][i0 + (o) * s[a]] : 0.0)
+              GRAD1(p, n[a], FV(0), FV(-1), FV(-2), FV(-3), FV(-4), FV(1), FV(2), FV(3), FV(4), r);
+#undef FV
+            }
+            d[(c * 3 + a) * NN + i0] = r; /* slot: x_xi,x_eta,x_zeta,y_xi,... */
+          }
+        const double x_xi = d[0 * NN + i0], x_eta = d[1 * NN + i0], x_zeta = d[2 * NN + i0];
+        const double y_xi = d[3 * NN + i0], y_eta = d[4 * NN + i0], y_zeta = d[5 * NN + i0];
+        const double z_xi = d[6 * NN + i0], z_eta = d[7 * NN + i0], z_zeta = d[8 * NN + i0];
+        /* src/metrics_kernel.cu:55-70 */
+        double Jac_a = -x_eta * y_xi * z_zeta + x_eta * y_zeta * z_xi
+                       + x_xi * y_eta * z_zeta - x_xi * y_zeta * z_eta
+                       - x_zeta * y_eta * z_xi + x_zeta * y_xi * z_eta;
+        long o = IDX(i, j, k);
+        const long S = g->S;
+        met[ORC_JAC * S + o] = Jac_a;
+        met[ORC_XI_X * S + o] = (y_eta * z_zeta - y_zeta * z_eta) / Jac_a;
+        met[ORC_XI_Y * S + o] = (-x_eta * z_zeta + x_zeta * z_eta) / Jac_a;
+        met[ORC_XI_Z * S + o] = (x_eta * y_zeta - x_zeta * y_eta) / Jac_a;
+        met[ORC_ETA_X * S + o] = (-y_xi * z_zeta + y_zeta * z_xi) / Jac_a;
+        met[ORC_ETA_Y * S + o] = (x_xi * z_zeta - x_zeta * z_xi) / Jac_a;
+        met[ORC_ETA_Z * S + o] = (-x_xi * y_zeta + x_zeta * y_xi) / Jac_a;
+        met[ORC_ZETA_X * S + o] = (-y_eta * z_xi + y_xi * z_eta) / Jac_a;
+        met[ORC_ZETA_Y * S + o] = (x_eta * z_xi - x_xi * z_eta) / Jac_a;
+        met[ORC_ZETA_Z * S + o] = (-x_eta * y_xi + x_xi * y_eta) / Jac_a;
+        if (!(Jac_a > 0.0)) bad |= 1;
+      }
+  free(d);
+  /* ghost extension */
+#pragma omp parallel for schedule(static)
+  for (int k = -G; k < KM + G; k++)
+    for (int j = -G; j < JM + G; j++)
+      for (int i = -G; i < IM + G; i++) {
+        if (i >= 0 && i < IM && j >= 0 && j < JM && k >= 0 && k < KM) continue;
+        long o = IDX(i, j, k);
+        long s = IDX(ghost_src(g, 0, i), ghost_src(g, 1, j), ghost_src(g, 2, k));
+        for (int c = 0; c < 10; c++) met[c * g->S + o] = met[c * g->S + s];
+      }
+  return bad ? 1 : 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* conservative -> primitive at one node, src/primitive_bc_kernel.cu:63-77 */
+static void prim_from_q(const orc_grid* g, const double* Q, double* prim, long qi, long pi) {
+  const long S = g->S;
+  const double rr = Q[qi];
+  const double uu = Q[S + qi] / rr;
+  const double vv = Q[2 * S + qi] / rr;
+  const double ww = Q[3 * S + qi] / rr;
+  const double pp = g->gam1 * (Q[4 * S + qi] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
+  prim[pi] = rr; prim[S + pi] = uu; prim[2 * S + pi] = vv; prim[3 * S + pi] = ww;
+  prim[4 * S + pi] = pp;
+  if (g->viscous) prim[5 * S + pi] = g->gM2 * pp / rr;
+}
+
+static void copy_node(const orc_grid* g, double* A, int ncomp, long dst, long src) {
+  for (int c = 0; c < ncomp; c++) A[c * g->S + dst] = A[c * g->S + src];
+}
+
+/* src/primitive_bc_kernel.cu:13-494, run as ordered phases (interior
+ * primitives; xi faces; eta=0 writes; eta=0 primitives; eta=JM-1; zeta faces;
+ * periodic wraps).  The reference orders its phases only by block-local
+ * barriers (:187, :414); this restatement uses the intended global order. */
+void orc_bc_primitive(const orc_grid* g, double* Q, double* prim, const double* met) {
+  const int IM = g->IM, JM = g->JM, KM = g->KM;
+  const long S = g->S;
+  const double gam1 = g->gam1, p_inf = g->p_inf;
+  const int NGR = 2; /* ghost depth the reference's boundary rules write */
+  const int ilo = upd_lo(g, 0), ihi = upd_hi(g, 0);
+  const int jlo = upd_lo(g, 1), jhi = upd_hi(g, 1);
+  const int klo = upd_lo(g, 2), khi = upd_hi(g, 2);
+  const int np = g->viscous ? 6 : 5;
+
+  /* interior primitives :59-79 */
+#pragma omp parallel for schedule(static)
+  for (int k = klo; k <= khi; k++)
+    for (int j = jlo; j <= jhi; j++)
+      for (int i = ilo; i <= ihi; i++) prim_from_q(g, Q, prim, IDX(i, j, k), IDX(i, j, k));
+
+  /* xi faces :84-186 (far field: density, momentum from the neighbour, p_inf) */
+  if (!periodic(g, 0)) {
+#pragma omp parallel for schedule(static)
+    for (int k = klo; k <= khi; k++)
+      for (int j = jlo; j <= jhi; j++)
+        for (int side = 0; side < 2; side++) {
+          int src = side == 0 ? 1 : IM - 2;
+          long s = IDX(src, j, k);
+          double rho_1 = Q[s], rho_u = Q[S + s], rho_v = Q[2 * S + s], rho_w = Q[3 * S + s];
+          double rho_et = p_inf / gam1 + 0.5 * (rho_u * rho_u + rho_v * rho_v + rho_w * rho_w) / rho_1;
+          for (int li = 0; li <= NGR; li++) {
+            long d = side == 0 ? IDX(-li, j, k) : IDX(IM - 1 + li, j, k);
+            Q[d] = rho_1; Q[S + d] = rho_u; Q[2 * S + d] = rho_v; Q[3 * S + d] = rho_w;
+            Q[4 * S + d] = rho_et;
+          }
+          const double rr = rho_1, uu = rho_u / rr, vv = rho_v / rr, ww = rho_w / rr;
+          const double pp = gam1 * (rho_et - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
+          for (int li = 0; li <= NGR; li++) {
+            long d = side == 0 ? IDX(-li, j, k) : IDX(IM - 1 + li, j, k);
+            prim[d] = rr; prim[S + d] = uu; prim[2 * S + d] = vv; prim[3 * S + d] = ww;
+            prim[4 * S + d] = pp;
+            if (g->viscous) prim[5 * S + d] = g->gM2 * pp / rr;
+          }
+        }
+  }
+
+  if (!periodic(g, 1)) {
+    /* eta = 0 writes :190-352 (wake cut / wall) */
+#pragma omp parallel for schedule(static)
+    for (int k = klo; k <= khi; k++)
+      for (int i = 0; i < IM; i++) {
+        if ((long)g->i_begin - 1 > i) {
+          long a = IDX(i, 1, k);
+          double rr = Q[a], uu = Q[S + a] / rr, vv = Q[2 * S + a] / rr, ww = Q[3 * S + a] / rr;
+          double pp = gam1 * (Q[4 * S + a] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
+          long b = IDX(IM - 1 - i, 1, k);
+          double rr1 = Q[b], uu1 = Q[S + b] / rr1, vv1 = Q[2 * S + b] / rr1, ww1 = Q[3 * S + b] / rr1;
+          double pp1 = gam1 * (Q[4 * S + b] - 0.5 * rr1 * (uu1 * uu1 + vv1 * vv1 + ww1 * ww1));
+          rr += rr1; uu += uu1; vv += vv1; ww += ww1; pp += pp1;
+          rr *= 0.5; uu *= 0.5; vv *= 0.5; ww *= 0.5; pp *= 0.5;
+          long d = IDX(i, 0, k);
+          Q[d] = rr; Q[S + d] = rr * uu; Q[2 * S + d] = rr * vv; Q[3 * S + d] = rr * ww;
+          Q[4 * S + d] = pp / gam1 + 0.5 * rr * (uu * uu + vv * vv + ww * ww);
+          long d2 = IDX(IM - 1 - i, 0, k);
+          copy_node(g, Q, 5, d2, d);
+          for (int lj = 1; lj <= NGR; lj++) {
+            copy_node(g, Q, 5, IDX(i, -lj, k), IDX(IM - 1 - i, lj, k));
+            copy_node(g, Q, 5, IDX(IM - 1 - i, -lj, k), IDX(i, lj, k));
+          }
+        } else if ((long)g->i_end > i) {
+          if (g->viscous) {
+            /* no-slip wall :264-295 */
+            long a = IDX(i, 1, k);
+            double rho_1 = Q[a], rho_u = Q[S + a], rho_v = Q[2 * S + a], rho_w = Q[3 * S + a];
+            double rho_et = Q[4 * S + a];
+            double pp = gam1 * (rho_et - 0.5 * (rho_u * rho_u + rho_v * rho_v + rho_w * rho_w) / rho_1);
+            long d = IDX(i, 0, k);
+            Q[d] = rho_1; Q[S + d] = 0.0; Q[2 * S + d] = 0.0; Q[3 * S + d] = 0.0;
+            Q[4 * S + d] = pp / gam1;
+            for (int lj = 1; lj <= NGR; lj++) {
+              long s = IDX(i, lj, k), t = IDX(i, -lj, k);
+              Q[t] = Q[s]; Q[S + t] = -Q[S + s]; Q[2 * S + t] = -Q[2 * S + s];
+              Q[3 * S + t] = -Q[3 * S + s]; Q[4 * S + t] = Q[4 * S + s];
+            }
+          } else {
+            /* slip wall :297-349, including the reference's energy formula */
+            long a = IDX(i, 1, k);
+            double rr = Q[a], uu = Q[S + a] / rr, vv = Q[2 * S + a] / rr, ww = Q[3 * S + a] / rr;
+            double pp = gam1 * (Q[4 * S + a] - 0.5 * rr * (uu * uu + vv * vv + ww * ww));
+            long m = IDX(i, 0, k);
+            double t_x = met[ORC_ETA_Y * S + m];
+            double t_y = -met[ORC_ETA_X * S + m];
+            double t_abs = met[ORC_JAC * S + m];
+            t_x *= t_abs; t_y *= t_abs;
+            t_abs = sqrt(t_x * t_x + t_y * t_y);
+            t_x /= t_abs; t_y /= t_abs;
+            for (int lj = 1; lj <= NGR; lj++) {
+              long s = IDX(i, lj, k), t = IDX(i, -lj, k);
+              t_abs = Q[S + s] * t_x + Q[2 * S + s] * t_y;
+              Q[t] = Q[s];
+              Q[S + t] = 2 * t_abs * t_x - Q[S + s];
+              Q[2 * S + t] = 2 * t_abs * t_y - Q[2 * S + s];
+              Q[3 * S + t] = Q[3 * S + s];
+              Q[4 * S + t] = Q[4 * S + s]
+                - 0.5 * (Q[S + s] * Q[S + s] + Q[2 * S + s] * Q[2 * S + s] -
+                         Q[S + t] * Q[S + t] + Q[2 * S + t] * Q[2 * S + t]) / Q[t];
+            }
+            t_abs = uu * t_x + vv * t_y;
+            uu = t_abs * t_x;
+            vv = t_abs * t_y;
+            long d = IDX(i, 0, k);
+            Q[d] = rr; Q[S + d] = rr * uu; Q[2 * S + d] = rr * vv; Q[3 * S + d] = rr * ww;
+            Q[4 * S + d] = pp / gam1 + 0.5 * rr * (uu * uu + vv * vv + ww * ww);
+          }
+        }
+      }
+    /* eta = 0 primitives :354-376 (every i) */
+#pragma omp parallel for schedule(static)
+    for (int k = klo; k <= khi; k++)
+      for (int i = 0; i < IM; i++)
+        for (int lj = 0; lj <= NGR; lj++) prim_from_q(g, Q, prim, IDX(i, -lj, k), IDX(i, -lj, k));
+    /* eta = JM-1 :383-413: primitives at JM-1 and JM from Q(JM-1); Q frozen */
+#pragma omp parallel for schedule(static)
+    for (int k = klo; k <= khi; k++)
+      for (int i = 0; i < IM; i++)
+        for (int lj = 0; lj < NGR; lj++) prim_from_q(g, Q, prim, IDX(i, JM - 1, k), IDX(i, JM - 1 + lj, k));
+  }
+
+  if (!periodic(g, 2)) {
+    /* zeta faces :417-492 (copy), every i, j of the node range */
+#pragma omp parallel for schedule(static)
+    for (int j = 0; j < JM; j++)
+      for (int i = 0; i < IM; i++)
+        for (int side = 0; side < 2; side++) {
+          long s = side == 0 ? IDX(i, j, 1) : IDX(i, j, KM - 2);
+          for (int lk = 0; lk <= NGR; lk++) {
+            long d = side == 0 ? IDX(i, j, -lk) : IDX(i, j, KM - 1 + lk);
+            copy_node(g, Q, 5, d, s);
+          }
+          for (int lk = 0; lk <= NGR; lk++) {
+            long d = side == 0 ? IDX(i, j, -lk) : IDX(i, j, KM - 1 + lk);
+            prim_from_q(g, Q, prim, s, d);
+          }
+        }
+  }
+
+  /* periodic wraps, axis by axis over the whole ghosted extent of the other
+   * axes so edges and corners come out right */
+  for (int a = 0; a < 3; a++) {
+    if (!periodic(g, a)) continue;
+    int n = axis_n(g, a);
+#pragma omp parallel for schedule(static)
+    for (int k = -G; k < KM + G; k++)
+      for (int j = -G; j < JM + G; j++)
+        for (int i = -G; i < IM + G; i++) {
+          int pos[3] = {i, j, k};
+          int m = pos[a];
+          if (m >= 0 && m < n) continue;
+          int src[3] = {i, j, k};
+          src[a] = ((m % n) + n) % n;
+          long d = IDX(i, j, k), s = IDX(src[0], src[1], src[2]);
+          copy_node(g, Q, 5, d, s);
+          copy_node(g, prim, np, d, s);
+        }
+  }
+}
+
+/* prim -> conservative over the ghosted extent, src/conservative_kernel.cu:13-48 */
+void orc_conservative(const orc_grid* g, const double* prim, double* Q) {
+  const long S = g->S;
+#pragma omp parallel for schedule(static)
+  for (long o = 0; o < S; o++) {
+    const double rr = prim[o], uu = prim[S + o], vv = prim[2 * S + o], ww = prim[3 * S + o];
+    const double pp = prim[4 * S + o];
+    Q[o] = rr; Q[S + o] = rr * uu; Q[2 * S + o] = rr * vv; Q[3 * S + o] = rr * ww;
+    Q[4 * S + o] = pp / g->gam1 + 0.5 * rr * (uu * uu + vv * vv + ww * ww);
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* src/calc_numerical_flux.cu:15-74 + src/rec_riemann_kernel.cu:15-247.  The
+ * flux of the face between node m and m+1 along direction d is stored at
+ * flux[(d*5 + c)*S + idx(node m)].  Faces: 0..N-2 (reference axis) or -1..N-1
+ * (periodic); transverse nodes: the update range. */
+void orc_numerical_flux(const orc_grid* g, const double* Q, const double* prim,
+                        const double* met, double* flux) {
+  const long S = g->S;
+  for (int d = 0; d < 3; d++) {
+    int lo[3], hi[3];
+    for (int a = 0; a < 3; a++) { lo[a] = upd_lo(g, a); hi[a] = upd_hi(g, a); }
+    lo[d] = periodic(g, d) ? -1 : 0;
+    hi[d] = axis_n(g, d) - 2 + (periodic(g, d) ? 1 : 0);
+    long st = d == 0 ? 1 : (d == 1 ? g->PX : g->PX * g->PY);
+    const double* nX = met + (3 * d + 0) * S;
+    const double* nY = met + (3 * d + 1) * S;
+    const double* nZ = met + (3 * d + 2) * S;
+    const double* J = met + ORC_JAC * S;
+#pragma omp parallel for schedule(static)
+    for (int k = lo[2]; k <= hi[2]; k++)
+      for (int j = lo[1]; j <= hi[1]; j++)
+        for (int i = lo[0]; i <= hi[0]; i++) {
+          long a = IDX(i, j, k), b = a + st;
+          double Jac_a = J[a], Jac_b = J[b];
+          double n_x = 0.5 * (nX[a] * Jac_a + nX[b] * Jac_b);
+          double n_y = 0.5 * (nY[a] * Jac_a + nY[b] * Jac_b);
+          double n_z = 0.5 * (nZ[a] * Jac_a + nZ[b] * Jac_b);
+          double pl[5], pr[5], q[30], f[5];
+          for (int c = 0; c < 5; c++) { pl[c] = prim[c * S + a]; pr[c] = prim[c * S + b]; }
+          for (int s = 0; s < 6; s++)
+            for (int c = 0; c < 5; c++) q[s * 5 + c] = Q[c * S + a + (s - 2) * st];
+          orc_face_flux(g, q, pl, pr, n_x, n_y, n_z, f);
+          for (int c = 0; c < 5; c++) flux[(d * 5 + c) * S + a] = f[c];
+        }
+  }
+}
+
+/* ------------------------------------------------------------------------ */
+/* Viscous terms and their divergence:
+ *   src/calc_viscous_terms.cu:16-88 (gradient_wg_kernel x4, viscous_terms_kernel)
+ *   src/calc_viscous_flux_contribution.cu:16-69 (viscous_flux_kernel,
+ *   viscous_contribution_kernel).
+ * Gradients, mu, tau, q and Ev/Fv/Gv live on the extended node range
+ * [-2, N+1] per axis; on a reference axis the two outermost layers use the
+ * one-sided stencils of gradient_wg_kernel (:162-253), on a periodic axis the
+ * central stencil reads the wrapped ghosts.  mu (Sutherland, unscaled) is
+ * written to mu[] over that range.  dfv[(n)*S + idx] for n = 0..3 at the
+ * update nodes. */
+void orc_viscous(const orc_grid* g, const double* prim, const double* met,
+                 double* mu, double* dfv) {
+  const long S = g->S;
+  const int IM = g->IM, JM = g->JM, KM = g->KM;
+  const int E = 2; /* extension of the viscous node range */
+  double* gr = (double*)malloc(sizeof(double) * 12 * S);   /* u,v,w,T x xi,eta,zeta */
+  double* tq = (double*)malloc(sizeof(double) * 9 * S);    /* 6 tau + 3 q */
+  double* ev = (double*)malloc(sizeof(double) * 12 * S);   /* Ev,Fv,Gv x 4 */
+  if (!gr || !tq || !ev) { free(gr); free(tq); free(ev); return; }
+  long st[3] = {1, g->PX, g->PX * g->PY};
+  int n[3] = {IM, JM, KM};
+
+#pragma omp parallel for schedule(static)
+  for (int k = -E; k < KM + E; k++)
+    for (int j = -E; j < JM + E; j++)
+      for (int i = -E; i < IM + E; i++) {
+        long o = IDX(i, j, k);
+        int pos[3] = {i, j, k};
+        for (int f = 0; f < 4; f++) {
+          const double* F = prim + (f == 3 ? ORC_T : ORC_U + f) * S;
+          for (int a = 0; a < 3; a++) {
+            double r = 0.0;
+            if (periodic(g, a)) {
+              r = central4(F[o - 2 * st[a]], F[o - st[a]], F[o + st[a]], F[o + 2 * st[a]]);
+            } else {
+              int p = pos[a] + E, ne = n[a] + 2 * E;
+              /* only offsets inside the extended range are read */
+#define FV(q) ((p + (q) >= 0 && p + (q) < ne) ? F[o + (q) * st[a]] : 0.0)
+              GRAD1(p, ne, FV(0), FV(-1), FV(-2), FV(-3), FV(-4), FV(1), FV(2), FV(3), FV(4), r);
+#undef FV
+            }
+            gr[(f * 3 + a) * S + o] = r;
+          }
+        }
+        /* src/viscous_terms_kernel.cu:59-105 */
+        const double T_l = prim[ORC_T * S + o];
+        const double C_T_inf = g->C_T_inf;
+        double mu_l = (1.0 + C_T_inf) / (T_l + C_T_inf) * sqrt(T_l * T_l * T_l);
+        mu[o] = mu_l;
+        const double xi_x_l = met[ORC_XI_X * S + o], xi_y_l = met[ORC_XI_Y * S + o], xi_z_l = met[ORC_XI_Z * S + o];
+        const double eta_x_l = met[ORC_ETA_X * S + o], eta_y_l = met[ORC_ETA_Y * S + o], eta_z_l = met[ORC_ETA_Z * S + o];
+        const double zeta_x_l = met[ORC_ZETA_X * S + o], zeta_y_l = met[ORC_ZETA_Y * S + o], zeta_z_l = met[ORC_ZETA_Z * S + o];
+        double u_xi_l = gr[0 * S + o], u_eta_l = gr[1 * S + o], u_zeta_l = gr[2 * S + o];
+        const double v_xi_l = gr[3 * S + o], v_eta_l = gr[4 * S + o], v_zeta_l = gr[5 * S + o];
+        const double w_xi_l = gr[6 * S + o], w_eta_l = gr[7 * S + o], w_zeta_l = gr[8 * S + o];
+        mu_l *= g->Re_inv;
+        tq[0 * S + o] = (2.0/3.0)*mu_l*(2*eta_x_l*u_eta_l - eta_y_l*v_eta_l - eta_z_l*w_eta_l + 2*u_xi_l*xi_x_l + 2*u_zeta_l*zeta_x_l - v_xi_l*xi_y_l - v_zeta_l*zeta_y_l - w_xi_l*xi_z_l - w_zeta_l*zeta_z_l);
+        tq[3 * S + o] = mu_l*(eta_x_l*v_eta_l + eta_y_l*u_eta_l + u_xi_l*xi_y_l + u_zeta_l*zeta_y_l + v_xi_l*xi_x_l + v_zeta_l*zeta_x_l);
+        tq[1 * S + o] = (2.0/3.0)*mu_l*(-eta_x_l*u_eta_l + 2*eta_y_l*v_eta_l - eta_z_l*w_eta_l - u_xi_l*xi_x_l - u_zeta_l*zeta_x_l + 2*v_xi_l*xi_y_l + 2*v_zeta_l*zeta_y_l - w_xi_l*xi_z_l - w_zeta_l*zeta_z_l);
+        tq[4 * S + o] = mu_l*(eta_x_l*w_eta_l + eta_z_l*u_eta_l + u_xi_l*xi_z_l + u_zeta_l*zeta_z_l + w_xi_l*xi_x_l + w_zeta_l*zeta_x_l);
+        tq[5 * S + o] = mu_l*(eta_y_l*w_eta_l + eta_z_l*v_eta_l + v_xi_l*xi_z_l + v_zeta_l*zeta_z_l + w_xi_l*xi_y_l + w_zeta_l*zeta_y_l);
+        tq[2 * S + o] = (2.0/3.0)*mu_l*(-eta_x_l*u_eta_l - eta_y_l*v_eta_l + 2*eta_z_l*w_eta_l - u_xi_l*xi_x_l - u_zeta_l*zeta_x_l - v_xi_l*xi_y_l - v_zeta_l*zeta_y_l + 2*w_xi_l*xi_z_l + 2*w_zeta_l*zeta_z_l);
+        mu_l *= -g->Pr_inv * g->gam1_inv / g->Mach2;
+        u_xi_l = gr[9 * S + o]; u_eta_l = gr[10 * S + o]; u_zeta_l = gr[11 * S + o];
+        tq[6 * S + o] = mu_l * (xi_x_l * u_xi_l + eta_x_l * u_eta_l + zeta_x_l * u_zeta_l);
+        tq[7 * S + o] = mu_l * (xi_y_l * u_xi_l + eta_y_l * u_eta_l + zeta_y_l * u_zeta_l);
+        tq[8 * S + o] = mu_l * (xi_z_l * u_xi_l + eta_z_l * u_eta_l + zeta_z_l * u_zeta_l);
+      }
+
+  /* src/viscous_flux_kernel.cu:54-110 (tau slots: xx,yy,zz,xy,xz,yz) */
+#pragma omp parallel for schedule(static)
+  for (int k = -E; k < KM + E; k++)
+    for (int j = -E; j < JM + E; j++)
+      for (int i = -E; i < IM + E; i++) {
+        long o = IDX(i, j, k);
+        const double Jac_a = met[ORC_JAC * S + o];
+        const double uu = prim[ORC_U * S + o], vv = prim[ORC_V * S + o], ww = prim[ORC_W * S + o];
+        const double txx = tq[0 * S + o], tyy = tq[1 * S + o], tzz = tq[2 * S + o];
+        const double txy = tq[3 * S + o], txz = tq[4 * S + o], tyz = tq[5 * S + o];
+        const double qx = tq[6 * S + o], qy = tq[7 * S + o], qz = tq[8 * S + o];
+        for (int d = 0; d < 3; d++) {
+          double n_x = met[(3 * d + 0) * S + o] * Jac_a;
+          double n_y = met[(3 * d + 1) * S + o] * Jac_a;
+          double n_z = met[(3 * d + 2) * S + o] * Jac_a;
+          ev[(d * 4 + 0) * S + o] = n_x * txx + n_y * txy + n_z * txz;
+          ev[(d * 4 + 1) * S + o] = n_x * txy + n_y * tyy + n_z * tyz;
+          ev[(d * 4 + 2) * S + o] = n_x * txz + n_y * tyz + n_z * tzz;
+          ev[(d * 4 + 3) * S + o] = n_x * (uu * txx + vv * txy + ww * txz - qx)
+                                   + n_y * (uu * txy + vv * tyy + ww * tyz - qy)
+                                   + n_z * (uu * txz + vv * tyz + ww * tzz - qz);
+        }
+      }
+
+  /* src/viscous_contribution_kernel.cu:35-67.  The reference reads the
+   * derivative along one axis at transverse offset -1 on both other axes
+   * (its Ev index (i+NG-3..i+NG+1, j, k) is node (i-2..i+2, j-1, k-1));
+   * visc_centered = 1 reads at offset 0. */
+  const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
+  const int sh = g->visc_centered ? 0 : -1;
+  const int ilo = upd_lo(g, 0), ihi = upd_hi(g, 0);
+  const int jlo = upd_lo(g, 1), jhi = upd_hi(g, 1);
+  const int klo = upd_lo(g, 2), khi = upd_hi(g, 2);
+#pragma omp parallel for schedule(static)
+  for (int k = klo; k <= khi; k++)
+    for (int j = jlo; j <= jhi; j++)
+      for (int i = ilo; i <= ihi; i++) {
+        for (int e = 0; e < 4; e++) {
+          const double* Ev = ev + (0 * 4 + e) * S;
+          const double* Fv = ev + (1 * 4 + e) * S;
+          const double* Gv = ev + (2 * 4 + e) * S;
+          long ox = IDX(i, j + sh, k + sh);
+          long oy = IDX(i + sh, j, k + sh);
+          long oz = IDX(i + sh, j + sh, k);
+          double df = c0 * Ev[ox - 2 * st[0]] + c1 * Ev[ox - st[0]] - c1 * Ev[ox + st[0]] - c0 * Ev[ox + 2 * st[0]];
+          df += c0 * Fv[oy - 2 * st[1]] + c1 * Fv[oy - st[1]] - c1 * Fv[oy + st[1]] - c0 * Fv[oy + 2 * st[1]];
+          df += c0 * Gv[oz - 2 * st[2]] + c1 * Gv[oz - st[2]] - c1 * Gv[oz + st[2]] - c0 * Gv[oz + 2 * st[2]];
+          dfv[e * S + IDX(i, j, k)] = df;
+        }
+      }
+  free(gr); free(tq); free(ev);
+}
+
+/* src/runge_kutta_kernel.cu:13-80 */
+void orc_update_rk3(const orc_grid* g, double dt, int stage, const double* met,
+                    const double* flux, const double* dfv, double* Q, double* Qp) {
+  const long S = g->S;
+  const long st[3] = {1, g->PX, g->PX * g->PY};
+  const int ilo = upd_lo(g, 0), ihi = upd_hi(g, 0);
+  const int jlo = upd_lo(g, 1), jhi = upd_hi(g, 1);
+  const int klo = upd_lo(g, 2), khi = upd_hi(g, 2);
+#pragma omp parallel for schedule(static)
+  for (int k = klo; k <= khi; k++)
+    for (int j = jlo; j <= jhi; j++)
+      for (int i = ilo; i <= ihi; i++) {
+        long o = IDX(i, j, k);
+        const double Jac_a = met[ORC_JAC * S + o];
+        for (int n = 0; n < NEQ; n++) {
+          const double* Ep = flux + (0 * 5 + n) * S;
+          const double* Fp = flux + (1 * 5 + n) * S;
+          const double* Gp = flux + (2 * 5 + n) * S;
+          double df = Ep[o] - Ep[o - st[0]];
+          df += Fp[o] - Fp[o - st[1]];
+          df += Gp[o] - Gp[o - st[2]];
+          if (g->viscous && n > 0) df -= dfv[(n - 1) * S + o];
+          df /= Jac_a;
+          long qi = n * S + o;
+          if (1 == stage) Qp[qi] = Q[qi] - dt * df;
+          else if (2 == stage) Qp[qi] = 0.75 * Q[qi] + 0.25 * (Qp[qi] - dt * df);
+          else if (3 == stage) Q[qi] = (Q[qi] + 2.0 * (Qp[qi] - dt * df)) / 3.0;
+        }
+      }
+}
+
+/* src/time_step_kernel.cu:13-96 + src/calc_dt.cu:18-62 (max over all nodes;
+ * NaN result -> returns NaN, the caller reports the error) */
+double orc_calc_dt(const orc_grid* g, const double* prim, const double* mu,
+                   const double* met) {
+  const long S = g->S;
+  double lam_max = -INFINITY;
+  int saw_nan = 0;
+#pragma omp parallel for schedule(static) reduction(max : lam_max) reduction(| : saw_nan)
+  for (int k = 0; k < g->KM; k++)
+    for (int j = 0; j < g->JM; j++)
+      for (int i = 0; i < g->IM; i++) {
+        long o = IDX(i, j, k);
+        double rr = prim[o], uu = prim[S + o], vv = prim[2 * S + o], ww = prim[3 * S + o];
+        double pp = prim[4 * S + o];
+        double c = sqrt(g->gamma * pp / rr);
+        double factor_v = 0.0;
+        if (g->viscous) factor_v = 2.0 * mu[o] * g->C_dt_v * sqrt(uu * uu + vv * vv + ww * ww) / c / rr;
+        double tsum = 0.0;
+        for (int d = 0; d < 3; d++) {
+          double n_x = met[(3 * d + 0) * S + o], n_y = met[(3 * d + 1) * S + o], n_z = met[(3 * d + 2) * S + o];
+          double qn = uu * n_x + vv * n_y + ww * n_z;
+          double n_abs = sqrt(n_x * n_x + n_y * n_y + n_z * n_z);
+          double t = fabs(qn) + c * n_abs;
+          if (g->viscous) t += n_abs * n_abs * factor_v;
+          if (d == 0) tsum = t; else tsum = tsum + t;
+        }
+        if (isnan(tsum)) saw_nan |= 1;
+        else if (tsum > lam_max) lam_max = tsum;
+      }
+  if (saw_nan) return NAN;
+  return g->CFL / lam_max;
+}
+
+/* Sutherland viscosity over the extended range, as the last
+ * calc_viscous_terms call leaves it (src/viscous_terms_kernel.cu:64) */
+static void orc_sutherland(const orc_grid* g, const double* prim, double* mu) {
+  const long S = g->S;
+#pragma omp parallel for schedule(static)
+  for (int k = -2; k < g->KM + 2; k++)
+    for (int j = -2; j < g->JM + 2; j++)
+      for (int i = -2; i < g->IM + 2; i++) {
+        long o = IDX(i, j, k);
+        const double T_l = prim[ORC_T * S + o], C = g->C_T_inf;
+        mu[o] = (1.0 + C) / (T_l + C) * sqrt(T_l * T_l * T_l);
+      }
+}
+
+/* One stage of src/Block3d.cpp:249-287.  Precondition: prim holds the
+ * primitives of the stage input (Q for stage 1, Qp otherwise). */
+int orc_rk3_stage(const orc_grid* g, double dt, int stage, double* Q, double* Qp,
+                  double* prim, const double* met, double* mu) {
+  const long S = g->S;
+  double* in = stage == 1 ? Q : Qp;
+  double* flux = (double*)calloc((size_t)15 * S, sizeof(double));
+  double* dfv = g->viscous ? (double*)calloc((size_t)4 * S, sizeof(double)) : NULL;
+  if (!flux || (g->viscous && !dfv)) { free(flux); free(dfv); return -1; }
+  orc_numerical_flux(g, in, prim, met, flux);
+  if (g->viscous) orc_viscous(g, prim, met, mu, dfv);
+  orc_update_rk3(g, dt, stage, met, flux, dfv, Q, Qp);
+  orc_bc_primitive(g, stage == 3 ? Q : Qp, prim, met);
+  free(flux); free(dfv);
+  return 0;
+}
+
+/* src/Block3d.cpp:237-287 (one pass of the time loop) */
+int orc_rk3_step(const orc_grid* g, double t_end, double* t_cur, double* dt_out,
+                 double* Q, double* Qp, double* prim, const double* met, double* mu) {
+  if (g->viscous) orc_sutherland(g, prim, mu);
+  double dt = orc_calc_dt(g, prim, mu, met);
+  if (isnan(dt)) return 2;
+  if (t_end - *t_cur < dt) dt = t_end - *t_cur;
+  *t_cur += dt;
+  if (dt_out) *dt_out = dt;
+  for (int s = 1; s <= 3; s++) {
+    int rc = orc_rk3_stage(g, dt, s, Q, Qp, prim, met, mu);
+    if (rc) return rc;
+  }
+  return 0;
+}

@@ -1,0 +1,70 @@
+"""The C ABI library without a GPU: it loads, exports every entry point
+include/cbns.h declares, and its parameter struct has the layout the header
+defines (checked against a gcc-compiled probe of the header)."""
+import ctypes
+import os
+import re
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from cbns3d_cuda_amd import block
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "cbns.h")
+
+
+def declared_symbols():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(cbns_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = block.lib()
+    names = declared_symbols()
+    assert len(names) >= 20
+    for n in names:
+        assert hasattr(L, n), n
+    assert sorted(block.EXPORTS) == names
+
+
+def test_params_layout_matches_header():
+    src = r"""
+    #include <stdio.h>
+    #include <stddef.h>
+    #include "cbns.h"
+    int main(void) {
+      printf("%zu %zu %zu %zu %zu\n", sizeof(cbns_params), offsetof(cbns_params, bc),
+             offsetof(cbns_params, CFL), offsetof(cbns_params, T_inf),
+             offsetof(cbns_params, angle_attack));
+      return 0;
+    }
+    """
+    with tempfile.TemporaryDirectory() as d:
+        c = os.path.join(d, "p.c")
+        exe = os.path.join(d, "p")
+        open(c, "w").write(src)
+        subprocess.run(["gcc", "-I", os.path.join(ROOT, "include"), c, "-o", exe], check=True)
+        got = [int(t) for t in subprocess.run([exe], capture_output=True, text=True,
+                                               check=True).stdout.split()]
+    P = block.Params
+    exp = [ctypes.sizeof(P), P.bc.offset, P.CFL.offset, P.T_inf.offset, P.angle_attack.offset]
+    assert got == exp
+
+
+def test_oracle_params_layout_matches_library(oracle):
+    a, b = oracle.Params, block.Params
+    assert ctypes.sizeof(a) == ctypes.sizeof(b)
+    for (na, _), (nb, _) in zip(a._fields_, b._fields_):
+        assert na == nb and getattr(a, na).offset == getattr(b, nb).offset
+
+
+def test_case_meshes_have_positive_jacobian(oracle):
+    from cbns3d_cuda_amd import cases
+    from helpers import oracle_block
+    for case in (cases.isentropic_vortex(16), cases.taylor_green(16), cases.c_grid_channel()):
+        o = oracle_block(oracle, case)
+        assert o.inner(o.met[9]).min() > 0.0, case.name
