@@ -157,7 +157,7 @@ int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, d
       box.n[a] = upd_hi(c, a) - upd_lo(c, a) + 1;
     }
     box.lo[0] = periodic(c, 0) ? -1 : 0;
-    box.n[0] = c.IM - 1 + (periodic(c, 0) ? 1 : 0);
+    box.n[0] = c.IM - 1 + (periodic(c, 0) ? 2 : 0);  // faces -1..N-1 or 0..N-2
     ProfScope ps(b, CBNS_K_XFACES);
     k_xfaces<<<box_blocks(box, 256), 256, 0, b->stream>>>(c, box, qin, b->met, fx);
     CBNS_HIP(b, hipGetLastError());

@@ -215,7 +215,7 @@ int launch_numerical_flux(cbns_block* b, const double* Q) {
   for (int d = 0; d < 3; d++) {
     Box box = update_box(c);
     box.lo[d] = periodic(c, d) ? -1 : 0;
-    box.n[d] = axis_n(c, d) - 1 + (periodic(c, d) ? 1 : 0);
+    box.n[d] = axis_n(c, d) - 1 + (periodic(c, d) ? 2 : 0);  // faces -1..N-1 or 0..N-2
     k_face_flux<<<box_blocks(box, 256), 256, 0, b->stream>>>(c, box, d, Q, b->prim, b->met,
                                                              b->flux);
     CBNS_HIP(b, hipGetLastError());
