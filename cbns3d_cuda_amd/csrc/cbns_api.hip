@@ -115,7 +115,7 @@ int ensure_unfused(cbns_block* b) {
   if (!b->flux && (rc = dmalloc(b, &b->flux, 15 * S))) return rc;
   if (b->c.viscous) {
     if (!b->dfv && (rc = dmalloc(b, &b->dfv, 4 * S))) return rc;
-    if (!b->vtmp && (rc = dmalloc(b, &b->vtmp, 33 * S))) return rc;
+    if (!b->vtmp && (rc = dmalloc(b, &b->vtmp, 12 * S))) return rc;
   }
   return CBNS_OK;
 }

@@ -100,7 +100,7 @@ struct cbns_block {
   double* rhs = nullptr;     // 5*S, fused-stage accumulator
   double* flux = nullptr;    // 15*S (lazy, unfused path)
   double* dfv = nullptr;     // 4*S  (lazy)
-  double* vtmp = nullptr;    // 33*S (lazy, unfused viscous intermediates)
+  double* vtmp = nullptr;    // 12*S (lazy): viscous fluxes Ev,Fv,Gv
   double* red = nullptr;     // reduction scratch
   int red_n = 0;
   double* h_scalar = nullptr;  // pinned host scalars [4]
@@ -141,6 +141,7 @@ int launch_conservative_all(cbns_block* b, double* Q);
 int launch_freestream(cbns_block* b);
 int launch_numerical_flux(cbns_block* b, const double* Q);
 int launch_viscous_unfused(cbns_block* b);
+int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode);
 int launch_update_rk3(cbns_block* b, double dt, int stage);
 int launch_calc_dt(cbns_block* b, double* dt_host);
 int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout);

@@ -7,9 +7,14 @@
 //     dot products, rows 3..4 are the tangent projections;
 //   - R (right eigenvectors) is applied in closed form;
 //   - each WENO5 evaluation uses one division (weights multiplied through by
-//     IS1^2 IS2^2 IS3^2) instead of seven;
+//     IS1^2 IS2^2 IS3^2) instead of seven, and the left and right
+//     evaluations of a field share their two common smoothness terms;
 //   - 1/x is formed once and multiplied where the reference divides by the
-//     same x repeatedly.
+//     same x repeatedly;
+//   - reciprocals and reciprocal square roots start from the hardware
+//     v_rcp_f64 / v_rsq_f64 estimate and take two Newton steps (about 1 ulp;
+//     an IEEE f64 divide or sqrt costs ~11-12 VALU instructions on gfx950,
+//     this ~5-7).  Arguments are positive and finite wherever they are used.
 // The result equals the reference formula up to rounding (the stage parity
 // bar is 1e-12 relative; tests/test_gpu_parity.py).
 #pragma once
@@ -17,6 +22,26 @@
 #include <hip/hip_runtime.h>
 
 namespace cbns {
+
+__device__ __forceinline__ double fast_rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = fma(-x, r, 1.0);
+  r = fma(r, e, r);
+  e = fma(-x, r, 1.0);
+  return fma(r, e, r);
+}
+__device__ __forceinline__ float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
+
+// 1/sqrt(x): y <- y + y (1/2 - x y^2 / 2), twice
+__device__ __forceinline__ double fast_rsq(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  double t = fma(-h * y, y, 0.5);
+  y = fma(y, t, y);
+  t = fma(-h * y, y, 0.5);
+  return fma(y, t, y);
+}
+__device__ __forceinline__ float fast_rsq(float x) { return __builtin_amdgcn_rsqf(x); }
 
 template <class T>
 struct GasT {
@@ -44,7 +69,7 @@ __device__ __forceinline__ T weno5(T a, T b, T c, T d, T e) {
   const T f1 = T(2) * a - T(7) * b + T(11) * c;
   const T f2 = -b + T(5) * c + T(2) * d;
   const T f3 = T(2) * c + T(5) * d - e;
-  return (w1 * f1 + w2 * f2 + w3 * f3) / (T(6) * (w1 + w2 + w3));
+  return (w1 * f1 + w2 * f2 + w3 * f3) * fast_rcp(T(6) * (w1 + w2 + w3));
 }
 
 // Roe-averaged characteristic basis of a face, src/calc_eigenvectors_roe.cu:28-150.
@@ -61,9 +86,9 @@ __device__ __forceinline__ void roe_basis(const GasT<T>& g, T rL, T uL, T vL, T 
                                           Basis<T>& B) {
   T l_x = T(0), l_y = n_z, l_z = -n_y;
   T m_x, m_y, m_z;
+  // |l| > 1e-8 of the reference, compared squared
   T l2 = l_y * l_y + l_z * l_z;
-  T labs = sqrt(l2);
-  if (labs > T(1.0e-8)) {
+  if (l2 > T(1.0e-16)) {
     m_x = -n_y * n_y - n_z * n_z;
     m_y = n_x * n_y;
     m_z = n_x * n_z;
@@ -72,30 +97,32 @@ __device__ __forceinline__ void roe_basis(const GasT<T>& g, T rL, T uL, T vL, T 
     m_x = n_x * n_y;
     m_y = -n_x * n_x - n_z * n_z;
     m_z = n_y * n_z;
-    labs = sqrt(l_x * l_x + l_z * l_z);
+    l2 = l_x * l_x + l_z * l_z;
   }
-  const T inv_n = T(1) / sqrt(n_x * n_x + n_y * n_y + n_z * n_z);
+  const T inv_n = fast_rsq(n_x * n_x + n_y * n_y + n_z * n_z);
   B.nx = n_x * inv_n; B.ny = n_y * inv_n; B.nz = n_z * inv_n;
-  const T inv_l = T(1) / labs;
+  const T inv_l = fast_rsq(l2);
   B.lx = l_x * inv_l; B.ly = l_y * inv_l; B.lz = l_z * inv_l;
-  const T inv_m = T(1) / sqrt(m_x * m_x + m_y * m_y + m_z * m_z);
+  const T inv_m = fast_rsq(m_x * m_x + m_y * m_y + m_z * m_z);
   B.mx = m_x * inv_m; B.my = m_y * inv_m; B.mz = m_z * inv_m;
 
   // H = c^2/gam1 + |u|^2/2 with c^2 = gamma p / rho
-  const T HL = g.gamma * pL / rL * g.gam1_inv + T(0.5) * (uL * uL + vL * vL + wL * wL);
-  const T HR = g.gamma * pR / rR * g.gam1_inv + T(0.5) * (uR * uR + vR * vR + wR * wR);
-  const T ratio = sqrt(rR / rL);
-  const T inv1r = T(1) / (T(1) + ratio);
+  const T irL = fast_rcp(rL), irR = fast_rcp(rR);
+  const T HL = g.gamma * pL * irL * g.gam1_inv + T(0.5) * (uL * uL + vL * vL + wL * wL);
+  const T HR = g.gamma * pR * irR * g.gam1_inv + T(0.5) * (uR * uR + vR * vR + wR * wR);
+  const T ratio = rR * fast_rsq(rR * rL);  // sqrt(rR / rL)
+  const T inv1r = fast_rcp(T(1) + ratio);
   B.uu = (uL + ratio * uR) * inv1r;
   B.vv = (vL + ratio * vR) * inv1r;
   B.ww = (wL + ratio * wR) * inv1r;
   B.H = (HL + ratio * HR) * inv1r;
   B.q2 = B.uu * B.uu + B.vv * B.vv + B.ww * B.ww;
-  B.c = sqrt(g.gam1 * (B.H - T(0.5) * B.q2));
+  const T c2 = g.gam1 * (B.H - T(0.5) * B.q2);
+  const T inv_c = fast_rsq(c2);
+  B.c = c2 * inv_c;
   B.qn = B.uu * B.nx + B.vv * B.ny + B.ww * B.nz;
   B.ql = B.uu * B.lx + B.vv * B.ly + B.ww * B.lz;
   B.qm = B.uu * B.mx + B.vv * B.my + B.ww * B.mz;
-  const T inv_c = T(1) / B.c;
   B.o = T(0.5) * inv_c;
   B.K = T(0.5) * g.gam1 * inv_c * inv_c;
 }
@@ -125,7 +152,7 @@ __device__ __forceinline__ void char_to_prim(const GasT<T>& g, const Basis<T>& B
   const T m2 = B.vv * rho + B.ny * dl + B.ly * W[3] + B.my * W[4];
   const T m3 = B.ww * rho + B.nz * dl + B.lz * W[3] + B.mz * W[4];
   const T E = B.H * s + T(0.5) * B.q2 * W[1] + B.qn * dl + B.ql * W[3] + B.qm * W[4];
-  const T ir = T(1) / rho;
+  const T ir = fast_rcp(rho);
   r = rho;
   u = m1 * ir;
   v = m2 * ir;
@@ -137,13 +164,15 @@ __device__ __forceinline__ void char_to_prim(const GasT<T>& g, const Basis<T>& B
 template <class T>
 __device__ __forceinline__ void lf_flux(const GasT<T>& g, T rR, T uR, T vR, T wR, T pR, T rL,
                                         T uL, T vL, T wL, T pL, T nx, T ny, T nz, T f[5]) {
-  const T cR = sqrt(g.gamma * pR / rR);
+  const T aR = g.gamma * pR * fast_rcp(rR);
+  const T cR = aR * fast_rsq(aR);
   const T kR = T(0.5) * (uR * uR + vR * vR + wR * wR);
-  const T HR = cR * cR * g.gam1_inv + kR;
+  const T HR = aR * g.gam1_inv + kR;
   const T EtR = pR * g.gam1_inv + rR * kR;
-  const T cL = sqrt(g.gamma * pL / rL);
+  const T aL = g.gamma * pL * fast_rcp(rL);
+  const T cL = aL * fast_rsq(aL);
   const T kL = T(0.5) * (uL * uL + vL * vL + wL * wL);
-  const T HL = cL * cL * g.gam1_inv + kL;
+  const T HL = aL * g.gam1_inv + kL;
   const T EtL = pL * g.gam1_inv + rL * kL;
   const T qR = uR * nx + vR * ny + wR * nz;
   const T qL = uL * nx + vL * ny + wL * nz;
@@ -164,31 +193,86 @@ template <class T>
 __device__ __forceinline__ void q_to_prim(const GasT<T>& g, const T q[5], T& r, T& u, T& v, T& w,
                                           T& p) {
   r = q[0];
-  const T ir = T(1) / r;
+  const T ir = fast_rcp(r);
   u = q[1] * ir;
   v = q[2] * ir;
   w = q[3] * ir;
   p = g.gam1 * (q[4] - T(0.5) * r * (u * u + v * v + w * w));
 }
 
+// WENO5-JS on both sides of one face from the 6 projected stencil values
+// w0..w5 (nodes m-2..m+3): left = weno5(w0..w4), right = weno5(w5..w1).
+// The (13/12)(.)^2 terms of the two middle sub-stencils are the same
+// second differences on both sides and are formed once.
+template <class T>
+__device__ __forceinline__ void weno5_pair(T w0, T w1, T w2, T w3, T w4, T w5, T& left,
+                                           T& right) {
+  const T eps = T(1.0e-6), k13 = T(13.0 / 12.0), q = T(0.25);
+  const T dc = w1 - T(2) * w2 + w3;  // centred at m
+  const T dd = w2 - T(2) * w3 + w4;  // centred at m+1
+  const T tc = k13 * dc * dc, td = k13 * dd * dd;
+  {
+    const T d1 = w0 - T(2) * w1 + w2, d2 = w0 - T(4) * w1 + T(3) * w2;
+    const T d4 = w1 - w3, d6 = T(3) * w2 - T(4) * w3 + w4;
+    const T IS1 = k13 * d1 * d1 + q * d2 * d2 + eps;
+    const T IS2 = tc + q * d4 * d4 + eps;
+    const T IS3 = td + q * d6 * d6 + eps;
+    const T s1 = IS1 * IS1, s2 = IS2 * IS2, s3 = IS3 * IS3;
+    const T a1 = T(0.1) * (s2 * s3), a2 = T(0.6) * (s1 * s3), a3 = T(0.3) * (s1 * s2);
+    const T f1 = T(2) * w0 - T(7) * w1 + T(11) * w2;
+    const T f2 = -w1 + T(5) * w2 + T(2) * w3;
+    const T f3 = T(2) * w2 + T(5) * w3 - w4;
+    left = (a1 * f1 + a2 * f2 + a3 * f3) * fast_rcp(T(6) * (a1 + a2 + a3));
+  }
+  {
+    const T d1 = w5 - T(2) * w4 + w3, d2 = w5 - T(4) * w4 + T(3) * w3;
+    const T d4 = w4 - w2, d6 = T(3) * w3 - T(4) * w2 + w1;
+    const T IS1 = k13 * d1 * d1 + q * d2 * d2 + eps;
+    const T IS2 = td + q * d4 * d4 + eps;
+    const T IS3 = tc + q * d6 * d6 + eps;
+    const T s1 = IS1 * IS1, s2 = IS2 * IS2, s3 = IS3 * IS3;
+    const T a1 = T(0.1) * (s2 * s3), a2 = T(0.6) * (s1 * s3), a3 = T(0.3) * (s1 * s2);
+    const T f1 = T(2) * w5 - T(7) * w4 + T(11) * w3;
+    const T f2 = -w4 + T(5) * w3 + T(2) * w2;
+    const T f3 = T(2) * w3 + T(5) * w2 - w1;
+    right = (a1 * f1 + a2 * f2 + a3 * f3) * fast_rcp(T(6) * (a1 + a2 + a3));
+  }
+}
+
 // Full face: q[s][c] holds nodes m-2..m+3 (face between m and m+1); pl, pr
 // are the primitives of nodes m and m+1; (nx,ny,nz) the unnormalised face
-// vector 0.5 (J grad xi |_m + J grad xi |_{m+1}).
+// vector 0.5 (J grad xi |_m + J grad xi |_{m+1}).  Characteristic fields are
+// reconstructed one at a time so only six projected values are live.
 template <class T>
 __device__ __forceinline__ void face_flux(const GasT<T>& g, const T q[6][5], const T pl[5],
                                           const T pr[5], T nx, T ny, T nz, T f[5]) {
   Basis<T> B;
   roe_basis(g, pl[0], pl[1], pl[2], pl[3], pl[4], pr[0], pr[1], pr[2], pr[3], pr[4], nx, ny, nz,
             B);
-  T W[6][5];
+  // rows 0..2 of L q share A = K (E - u.m + q2/2 rho) and Bn = o (qn rho - n.m)
+  T A[6], Bn[6];
 #pragma unroll
-  for (int s = 0; s < 6; s++) char_project(B, q[s], W[s]);
-  T WL[5], WR[5];
-#pragma unroll
-  for (int m = 0; m < 5; m++) {
-    WL[m] = weno5(W[0][m], W[1][m], W[2][m], W[3][m], W[4][m]);
-    WR[m] = weno5(W[5][m], W[4][m], W[3][m], W[2][m], W[1][m]);
+  for (int s = 0; s < 6; s++) {
+    A[s] = B.K * (q[s][4] - (B.uu * q[s][1] + B.vv * q[s][2] + B.ww * q[s][3]) +
+                  T(0.5) * B.q2 * q[s][0]);
+    Bn[s] = B.o * (B.qn * q[s][0] - (B.nx * q[s][1] + B.ny * q[s][2] + B.nz * q[s][3]));
   }
+  T WL[5], WR[5];
+  weno5_pair(A[0] + Bn[0], A[1] + Bn[1], A[2] + Bn[2], A[3] + Bn[3], A[4] + Bn[4], A[5] + Bn[5],
+             WL[0], WR[0]);
+  weno5_pair(q[0][0] - T(2) * A[0], q[1][0] - T(2) * A[1], q[2][0] - T(2) * A[2],
+             q[3][0] - T(2) * A[3], q[4][0] - T(2) * A[4], q[5][0] - T(2) * A[5], WL[1], WR[1]);
+  weno5_pair(A[0] - Bn[0], A[1] - Bn[1], A[2] - Bn[2], A[3] - Bn[3], A[4] - Bn[4], A[5] - Bn[5],
+             WL[2], WR[2]);
+  T w[6];
+#pragma unroll
+  for (int s = 0; s < 6; s++)
+    w[s] = -B.ql * q[s][0] + B.lx * q[s][1] + B.ly * q[s][2] + B.lz * q[s][3];
+  weno5_pair(w[0], w[1], w[2], w[3], w[4], w[5], WL[3], WR[3]);
+#pragma unroll
+  for (int s = 0; s < 6; s++)
+    w[s] = -B.qm * q[s][0] + B.mx * q[s][1] + B.my * q[s][2] + B.mz * q[s][3];
+  weno5_pair(w[0], w[1], w[2], w[3], w[4], w[5], WL[4], WR[4]);
   T rL, uL, vL, wL, pL, rR, uR, vR, wR, pR;
   char_to_prim(g, B, WL, rL, uL, vL, wL, pL);
   char_to_prim(g, B, WR, rR, uR, vR, wR, pR);

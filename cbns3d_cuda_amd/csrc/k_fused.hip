@@ -6,10 +6,11 @@
 //                eta holding the 6-node stencil and the previous face flux in
 //                registers: each node is loaded once per sweep and each face
 //                is computed once.  Writes rhs = dE + dF.
-//   [viscous]    dfv from the viscous kernels
-//   k_march<2>   the same march along zeta; adds dG, subtracts the viscous
-//                divergence, divides by J and applies the Shu-Osher stage
-//                combination, writing the stage output.
+//   [viscous]    k_visc_fluxes + k_visc_div subtract the viscous divergence
+//                from rhs (k_viscous.hip)
+//   k_march<2>   the same march along zeta; adds dG, divides by J and
+//                applies the Shu-Osher stage combination, writing the stage
+//                output.
 //
 // Node primitives for the Roe average are recomputed from Q (the
 // primitive array holds exactly f(Q) at every node a face reads:
@@ -23,7 +24,16 @@ namespace {
 
 constexpr int MARCH_X = 64;  // lanes along i
 constexpr int MARCH_Y = 4;   // lines per workgroup along the other axis
-constexpr int CHUNK = 32;    // nodes per thread along the march axis
+#ifndef MARCH_CHUNK
+#define MARCH_CHUNK 32
+#endif
+constexpr int CHUNK = MARCH_CHUNK;  // nodes per thread along the march axis
+#ifndef WPE_X
+#define WPE_X 3  // waves per SIMD the x-face kernel is compiled for
+#endif
+#ifndef WPE_M
+#define WPE_M 2  // waves per SIMD the march kernels are compiled for
+#endif
 
 template <class T>
 __device__ __forceinline__ void load_q(const double* __restrict__ Q, long S, int o, T q[5]) {
@@ -31,7 +41,7 @@ __device__ __forceinline__ void load_q(const double* __restrict__ Q, long S, int
   for (int n = 0; n < 5; n++) q[n] = (T)Q[n * S + o];
 }
 
-__global__ void __launch_bounds__(256) k_xfaces(Consts c, Box box, const double* __restrict__ Q,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE_X, WPE_X))) k_xfaces(Consts c, Box box, const double* __restrict__ Q,
                                                 const double* __restrict__ met,
                                                 double* __restrict__ fx) {
   int i, j, k;
@@ -66,7 +76,7 @@ struct StageArgs {
 
 // AX = 1: march along eta, rhs = dE + dF.  AX = 2: march along zeta, finish.
 template <int AX>
-__global__ void __launch_bounds__(MARCH_X * MARCH_Y) k_march(Consts c, StageArgs sa,
+__global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves_per_eu(WPE_M, WPE_M))) k_march(Consts c, StageArgs sa,
                                                             const double* __restrict__ met) {
   const int ax_t = AX == 1 ? 2 : 1;  // transverse axis other than i
   const int i = upd_lo(c, 0) + blockIdx.x * MARCH_X + threadIdx.x;
@@ -175,9 +185,8 @@ int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, d
   }
   if (c.viscous) {
     ProfScope ps(b, CBNS_K_VISCOUS);
-    int rc = launch_viscous_unfused(b);
+    int rc = launch_viscous(b, b->vtmp, nullptr, b->rhs, 1);
     if (rc) return rc;
-    sa.dfv = b->dfv;
   }
   {
     const int nt = upd_hi(c, 1) - upd_lo(c, 1) + 1;

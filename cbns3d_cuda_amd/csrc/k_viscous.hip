@@ -1,0 +1,154 @@
+// k_viscous.hip -- viscous terms in two passes instead of the reference's six
+// (gradient_wg_kernel x4, viscous_terms_kernel, viscous_flux_kernel,
+// viscous_contribution_kernel; src/calc_viscous_terms.cu:16-88 and
+// src/calc_viscous_flux_contribution.cu:16-69):
+//
+//   k_visc_fluxes  per node of the viscous range [-2, N+1]^3: 4th-order
+//                  derivatives of u,v,w,T along xi,eta,zeta, Sutherland mu,
+//                  stress tau and heat flux q, contracted with J grad xi_d
+//                  into the 12 values Ev,Fv,Gv (4 each).  The 12 gradients,
+//                  6 stresses and 3 heat fluxes never reach HBM.
+//   k_visc_div     4th-order central divergence of Ev,Fv,Gv at the update
+//                  nodes, at the reference's transverse offset (-1) or
+//                  centred; written to dfv (reference decomposition) or
+//                  subtracted from the fused stage's right-hand side.
+#include "cbns_internal.hpp"
+
+namespace cbns {
+
+namespace {
+
+// 4th-order derivative along axis a at node m of the viscous range:
+// one-sided at the two outermost layers of a reference axis
+// (src/gradient_kernel.cu:162-253), central with wrapped ghosts otherwise.
+__device__ __forceinline__ double dgrad(const Consts& c, int a, int m, const double* __restrict__ f,
+                                        int st) {
+  if (!periodic(c, a)) {
+    const int p = m + 2, ne = axis_n(c, a) + 4;
+    if (0 == p)
+      return -25.0 / 12.0 * f[0] + 4.0 * f[st] - 3.0 * f[2 * st] + (4.0 / 3.0) * f[3 * st] -
+             0.25 * f[4 * st];
+    if (1 == p)
+      return -5.0 / 6.0 * f[0] - 0.25 * f[-st] + 1.5 * f[st] - 0.5 * f[2 * st] +
+             (1.0 / 12.0) * f[3 * st];
+    if (ne - 2 == p)
+      return (5.0 / 6.0) * f[0] - 1.0 / 12.0 * f[-3 * st] + 0.5 * f[-2 * st] - 1.5 * f[-st] +
+             0.25 * f[st];
+    if (ne - 1 == p)
+      return (25.0 / 12.0) * f[0] + 0.25 * f[-4 * st] - 4.0 / 3.0 * f[-3 * st] +
+             3.0 * f[-2 * st] - 4.0 * f[-st];
+  }
+  return (1.0 / 12.0) * f[-2 * st] - 2.0 / 3.0 * f[-st] + (2.0 / 3.0) * f[st] -
+         1.0 / 12.0 * f[2 * st];
+}
+
+__global__ void __launch_bounds__(256) k_visc_fluxes(Consts c, Box box,
+                                                     const double* __restrict__ prim,
+                                                     const double* __restrict__ met,
+                                                     double* __restrict__ mu,
+                                                     double* __restrict__ vf) {
+  int i, j, k;
+  if (!box_node(box, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  const long S = c.S;
+  const int o = idx(c, i, j, k);
+  const double* U = prim + P_U * S + o;
+  const double* V = prim + P_V * S + o;
+  const double* W = prim + P_W * S + o;
+  const double* T = prim + P_T * S + o;
+  const int sx = 1, sy = c.PX, sz = c.PXY;
+  double u_xi_l = dgrad(c, 0, i, U, sx), u_eta_l = dgrad(c, 1, j, U, sy), u_zeta_l = dgrad(c, 2, k, U, sz);
+  const double v_xi_l = dgrad(c, 0, i, V, sx), v_eta_l = dgrad(c, 1, j, V, sy), v_zeta_l = dgrad(c, 2, k, V, sz);
+  const double w_xi_l = dgrad(c, 0, i, W, sx), w_eta_l = dgrad(c, 1, j, W, sy), w_zeta_l = dgrad(c, 2, k, W, sz);
+  const double T_xi = dgrad(c, 0, i, T, sx), T_eta = dgrad(c, 1, j, T, sy), T_zeta = dgrad(c, 2, k, T, sz);
+
+  // src/viscous_terms_kernel.cu:59-105
+  const double T_l = T[0];
+  const double C_T_inf = c.C_T_inf;
+  double mu_l = (1.0 + C_T_inf) / (T_l + C_T_inf) * sqrt(T_l * T_l * T_l);
+  if (mu) mu[o] = mu_l;
+  const double xi_x_l = met[XI_X * S + o], xi_y_l = met[XI_Y * S + o], xi_z_l = met[XI_Z * S + o];
+  const double eta_x_l = met[ETA_X * S + o], eta_y_l = met[ETA_Y * S + o],
+               eta_z_l = met[ETA_Z * S + o];
+  const double zeta_x_l = met[ZETA_X * S + o], zeta_y_l = met[ZETA_Y * S + o],
+               zeta_z_l = met[ZETA_Z * S + o];
+  mu_l *= c.Re_inv;
+  const double txx = (2.0/3.0)*mu_l*(2*eta_x_l*u_eta_l - eta_y_l*v_eta_l - eta_z_l*w_eta_l + 2*u_xi_l*xi_x_l + 2*u_zeta_l*zeta_x_l - v_xi_l*xi_y_l - v_zeta_l*zeta_y_l - w_xi_l*xi_z_l - w_zeta_l*zeta_z_l);
+  const double txy = mu_l*(eta_x_l*v_eta_l + eta_y_l*u_eta_l + u_xi_l*xi_y_l + u_zeta_l*zeta_y_l + v_xi_l*xi_x_l + v_zeta_l*zeta_x_l);
+  const double tyy = (2.0/3.0)*mu_l*(-eta_x_l*u_eta_l + 2*eta_y_l*v_eta_l - eta_z_l*w_eta_l - u_xi_l*xi_x_l - u_zeta_l*zeta_x_l + 2*v_xi_l*xi_y_l + 2*v_zeta_l*zeta_y_l - w_xi_l*xi_z_l - w_zeta_l*zeta_z_l);
+  const double txz = mu_l*(eta_x_l*w_eta_l + eta_z_l*u_eta_l + u_xi_l*xi_z_l + u_zeta_l*zeta_z_l + w_xi_l*xi_x_l + w_zeta_l*zeta_x_l);
+  const double tyz = mu_l*(eta_y_l*w_eta_l + eta_z_l*v_eta_l + v_xi_l*xi_z_l + v_zeta_l*zeta_z_l + w_xi_l*xi_y_l + w_zeta_l*zeta_y_l);
+  const double tzz = (2.0/3.0)*mu_l*(-eta_x_l*u_eta_l - eta_y_l*v_eta_l + 2*eta_z_l*w_eta_l - u_xi_l*xi_x_l - u_zeta_l*zeta_x_l - v_xi_l*xi_y_l - v_zeta_l*zeta_y_l + 2*w_xi_l*xi_z_l + 2*w_zeta_l*zeta_z_l);
+  mu_l *= c.qcoef;
+  const double qx = mu_l * (xi_x_l * T_xi + eta_x_l * T_eta + zeta_x_l * T_zeta);
+  const double qy = mu_l * (xi_y_l * T_xi + eta_y_l * T_eta + zeta_y_l * T_zeta);
+  const double qz = mu_l * (xi_z_l * T_xi + eta_z_l * T_eta + zeta_z_l * T_zeta);
+
+  // src/viscous_flux_kernel.cu:57-110
+  const double Jac_a = met[JAC * S + o];
+  const double uu = U[0], vv = V[0], ww = W[0];
+  const double ex = uu * txx + vv * txy + ww * txz - qx;
+  const double ey = uu * txy + vv * tyy + ww * tyz - qy;
+  const double ez = uu * txz + vv * tyz + ww * tzz - qz;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double n_x = met[(3 * d + 0) * S + o] * Jac_a;
+    const double n_y = met[(3 * d + 1) * S + o] * Jac_a;
+    const double n_z = met[(3 * d + 2) * S + o] * Jac_a;
+    double* E = vf + (long)(4 * d) * S + o;
+    E[0] = n_x * txx + n_y * txy + n_z * txz;
+    E[S] = n_x * txy + n_y * tyy + n_z * tyz;
+    E[2 * S] = n_x * txz + n_y * tyz + n_z * tzz;
+    E[3 * S] = n_x * ex + n_y * ey + n_z * ez;
+  }
+}
+
+// src/viscous_contribution_kernel.cu:35-67.  mode 0: dfv[e] = div; mode 1:
+// out[(e+1)] -= div (the fused stage's right-hand side, momentum and energy).
+__global__ void __launch_bounds__(256) k_visc_div(Consts c, Box box, const double* __restrict__ vf,
+                                                  double* __restrict__ out, int mode) {
+  int i, j, k;
+  if (!box_node(box, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  const long S = c.S;
+  const int sh = c.visc_centered ? 0 : -1;
+  const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
+  const int ox = idx(c, i, j + sh, k + sh), oy = idx(c, i + sh, j, k + sh),
+            oz = idx(c, i + sh, j + sh, k);
+  const int sx = 1, sy = c.PX, sz = c.PXY;
+  const int o = idx(c, i, j, k);
+#pragma unroll
+  for (int e = 0; e < 4; e++) {
+    const double* Ev = vf + (long)e * S;
+    const double* Fv = vf + (long)(4 + e) * S;
+    const double* Gv = vf + (long)(8 + e) * S;
+    double df = c0 * Ev[ox - 2 * sx] + c1 * Ev[ox - sx] - c1 * Ev[ox + sx] - c0 * Ev[ox + 2 * sx];
+    df += c0 * Fv[oy - 2 * sy] + c1 * Fv[oy - sy] - c1 * Fv[oy + sy] - c0 * Fv[oy + 2 * sy];
+    df += c0 * Gv[oz - 2 * sz] + c1 * Gv[oz - sz] - c1 * Gv[oz + sz] - c0 * Gv[oz + 2 * sz];
+    if (mode == 0) out[e * S + o] = df;
+    else out[(e + 1) * S + o] -= df;
+  }
+}
+
+}  // namespace
+
+// vf: 12*S scratch; mu may be null.  mode 0 writes dfv (4*S) to `out`,
+// mode 1 subtracts the divergence from the right-hand side `out` (5*S).
+int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode) {
+  const Consts& c = b->c;
+  Box vb;
+  for (int a = 0; a < 3; a++) {
+    vb.lo[a] = -2;
+    vb.n[a] = axis_n(c, a) + 4;
+  }
+  k_visc_fluxes<<<box_blocks(vb, 256), 256, 0, b->stream>>>(c, vb, b->prim, b->met, mu, vf);
+  CBNS_HIP(b, hipGetLastError());
+  Box ub;
+  for (int a = 0; a < 3; a++) {
+    ub.lo[a] = upd_lo(c, a);
+    ub.n[a] = upd_hi(c, a) - upd_lo(c, a) + 1;
+  }
+  k_visc_div<<<box_blocks(ub, 256), 256, 0, b->stream>>>(c, ub, vf, out, mode);
+  CBNS_HIP(b, hipGetLastError());
+  return CBNS_OK;
+}
+
+}  // namespace cbns

@@ -68,3 +68,16 @@ def test_case_meshes_have_positive_jacobian(oracle):
     for case in (cases.isentropic_vortex(16), cases.taylor_green(16), cases.c_grid_channel()):
         o = oracle_block(oracle, case)
         assert o.inner(o.met[9]).min() > 0.0, case.name
+
+
+def test_taylor_green_initial_diagnostics(oracle):
+    """<rho |u|^2>/2 = 1/8 and <rho |omega|^2>/2 = 3/8 for the Taylor-Green
+    initial field (rho = 1 + O(M^2)), on the perturbed curvilinear mesh."""
+    from cbns3d_cuda_amd import cases, diagnostics
+    from helpers import oracle_block
+    case = cases.taylor_green(48)
+    o = oracle_block(oracle, case)
+    met = o.inner(o.met)
+    prim = case.prim
+    assert diagnostics.kinetic_energy(prim, met) == pytest.approx(0.125, rel=2e-3)
+    assert diagnostics.enstrophy(prim, met) == pytest.approx(0.375, rel=5e-3)
