@@ -26,14 +26,46 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 
 # Algorithmic work per unit.  Face flux: FP64 operations of one face of the
-# reference's method (Roe basis + 2 x characteristic WENO5 + LF flux),
-# counted on the oracle with an operation-counting scalar (oracle/opcount.c);
-# until that count is available the hand count 1610 is used.
-FLOPS_PER_FACE = 1610.0
-# HBM bytes per node of one sweep: Q (5) and the sweep's metric (3 + J) read,
-# rhs written (eta) or rhs read + Q_out written (+ Q_n for stages 2-3, zeta).
-BYTES_PER_NODE = {"xfaces": (5 + 4 + 5) * 8, "march_eta": (5 + 4 + 5 + 5) * 8,
-                  "march_zeta": (5 + 4 + 5 + 5 + 5 + 4 + 1) * 8}
+# reference's method (normal average, Roe basis, 2 x characteristic WENO5,
+# LF flux) = 673 add/sub + 795 mul + 102 div + 10 sqrt, counted on the oracle
+# with an operation-counting scalar (make -C oracle opcount && oracle/opcount).
+FLOPS_PER_FACE = 1580.0
+# Algorithmic HBM bytes of the memory-bound kernels (compulsory traffic, each
+# value moved once): viscous fluxes read u,v,w,T and the 10 metrics and write
+# Ev,Fv,Gv (12) per node of the viscous range (N+4)^3; the divergence reads
+# the 12 fluxes and read-modify-writes the 4 momentum/energy rhs components
+# per updated node; dt reads 6 primitives and 9 metrics per node.
+VISC_FLUX_BYTES_PER_NODE = (4 + 10 + 12) * 8
+VISC_DIV_BYTES_PER_NODE = (12 + 2 * 4) * 8
+DT_BYTES_PER_NODE = (6 + 9) * 8
+
+
+def kernel_work(name, N):
+    """(kind, units of work per launch) for the per-kernel roofline."""
+    faces = N * N * (N + 1)
+    if name in ("xfaces", "march_eta", "march_zeta"):
+        return "flop", faces * FLOPS_PER_FACE
+    if name == "visc_flux":
+        return "byte", (N + 4) ** 3 * VISC_FLUX_BYTES_PER_NODE
+    if name == "visc_div":
+        return "byte", N ** 3 * VISC_DIV_BYTES_PER_NODE
+    if name == "dt":
+        return "byte", N ** 3 * DT_BYTES_PER_NODE
+    # ghost shell of 4 layers: Q (5) + primitives (6) read and written
+    return "byte", ((N + 8) ** 3 - N ** 3) * 11 * 16
+
+
+def roofline_of(name, ms, n, N):
+    kind, work = kernel_work(name, N)
+    avg_s = ms / n / 1e3
+    if kind == "flop":
+        ach = work / avg_s / 1e12
+        return {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": name, "avg_ms": ms / n,
+                "note": "FP64 VALU-bound face-flux sweep; peak = MI355X FP64 (vector = matrix)"}
+    ach = work / avg_s / 1e9
+    return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": name, "avg_ms": ms / n}
 
 
 def log(*a):
@@ -113,24 +145,12 @@ def main():
         if n:
             log("  %-11s %8.3f ms total  %5d launches  %.4f ms avg" % (k, ms, n, ms / n))
 
-    # dominant kernel among the flux sweeps and its algorithmic work per launch
-    dom = max(kt, key=lambda k: kt[k][0])
-    ms, n = kt[dom]
-    avg_s = ms / n / 1e3
+    # dominant kernel and its algorithmic work per launch; every kernel's
+    # roofline is listed beside it
     nodes = N ** 3
-    if dom in ("xfaces", "march_eta", "march_zeta"):
-        faces = N * N * (N + 1)
-        flops = faces * FLOPS_PER_FACE
-        achieved = flops / avg_s / 1e12
-        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS,
-                "unit": "TFLOP/s", "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                "kernel": dom, "avg_ms": ms / n,
-                "note": "FP64 VALU-bound face-flux sweep; peak = MI355X FP64 (vector = matrix)"}
-    else:
-        byts = nodes * 8 * 40
-        achieved = byts / avg_s / 1e9
-        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                "frac": achieved / PEAK_HBM_GBS, "traffic": None, "kernel": dom, "avg_ms": ms / n}
+    dom = max(kt, key=lambda k: kt[k][0])
+    roof = roofline_of(dom, kt[dom][0], kt[dom][1], N)
+    per_kernel = {k: roofline_of(k, ms, n, N) for k, (ms, n) in kt.items() if n}
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -156,6 +176,8 @@ def main():
         "roofline": roof,
         "cpu_baseline": cpu,
         "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in kt.items() if v[1]},
+        "kernel_roofline": {k: {"frac": r["frac"], "achieved": r["achieved"], "unit": r["unit"]}
+                            for k, r in per_kernel.items()},
     }
     print(json.dumps(out), flush=True)
 

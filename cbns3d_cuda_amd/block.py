@@ -71,7 +71,7 @@ EXPORTS = [
     "cbns_rk3_step", "cbns_set_form", "cbns_get_field", "cbns_set_field",
     "cbns_synchronize", "cbns_field_size", "cbns_set_profiling", "cbns_kernel_time",
 ]
-KERNELS = ["xfaces", "march_eta", "march_zeta", "viscous", "bc", "dt"]
+KERNELS = ["xfaces", "march_eta", "march_zeta", "visc_flux", "visc_div", "bc", "dt"]
 
 
 def lib():

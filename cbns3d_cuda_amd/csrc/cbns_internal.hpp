@@ -54,22 +54,86 @@ __host__ __device__ inline int upd_hi(const Consts& c, int a) {
 enum { XI_X = 0, XI_Y, XI_Z, ETA_X, ETA_Y, ETA_Z, ZETA_X, ZETA_Y, ZETA_Z, JAC };
 enum { P_RHO = 0, P_U, P_V, P_W, P_P, P_T };
 
-// A 3-D box of nodes [lo, lo+n) flattened onto a 1-D grid.
+// A 3-D box of nodes [lo, lo+n).
 struct Box {
   int lo[3];
   int n[3];
   __host__ __device__ long count() const { return (long)n[0] * n[1] * n[2]; }
 };
+// Small boxes (boundary rules) run on a flattened 1-D grid; every count here
+// is below 2^31 (field sizes are checked at cbns_create), so the decode uses
+// 32-bit integer division.
 __device__ inline bool box_node(const Box& b, long t, int& i, int& j, int& k) {
   if (t >= b.count()) return false;
-  i = b.lo[0] + (int)(t % b.n[0]);
-  long r = t / b.n[0];
-  j = b.lo[1] + (int)(r % b.n[1]);
-  k = b.lo[2] + (int)(r / b.n[1]);
+  const unsigned u = (unsigned)t;
+  const unsigned r = u / (unsigned)b.n[0];
+  i = b.lo[0] + (int)(u - r * (unsigned)b.n[0]);
+  const unsigned q = r / (unsigned)b.n[1];
+  j = b.lo[1] + (int)(r - q * (unsigned)b.n[1]);
+  k = b.lo[2] + (int)q;
   return true;
 }
 inline unsigned box_blocks(const Box& b, int threads) {
   return (unsigned)((b.count() + threads - 1) / threads);
+}
+// Large boxes run on a 3-D grid: 64 lanes along i, 4 rows along j, one
+// plane per grid z, so no index division at all.
+constexpr int BOX_X = 64, BOX_Y = 4;
+inline dim3 box_grid(const Box& b) {
+  return dim3((unsigned)((b.n[0] + BOX_X - 1) / BOX_X), (unsigned)((b.n[1] + BOX_Y - 1) / BOX_Y),
+              (unsigned)b.n[2]);
+}
+inline dim3 box_block() { return dim3(BOX_X, BOX_Y, 1); }
+__device__ inline bool box_node3(const Box& b, int& i, int& j, int& k) {
+  const int ti = blockIdx.x * BOX_X + threadIdx.x, tj = blockIdx.y * BOX_Y + threadIdx.y;
+  if (ti >= b.n[0] || tj >= b.n[1]) return false;
+  i = b.lo[0] + ti;
+  j = b.lo[1] + tj;
+  k = b.lo[2] + (int)blockIdx.z;
+  return true;
+}
+
+// Tile march: 64 x 4 tiles of (i, j), each thread walking a chunk of
+// `chunk` nodes along k.  Workgroups are handed to the 8 XCDs round-robin and
+// every XCD has its own L2, so the flat block id is remapped to give each XCD
+// one contiguous run of tiles (neighbouring rows and chunks then share that
+// XCD's L2).  The grid is rounded up to a multiple of 8 blocks; blocks past
+// the tile count return.
+#ifndef CBNS_N_XCD
+#define CBNS_N_XCD 8
+#endif
+constexpr int N_XCD = CBNS_N_XCD;
+struct TileMarch {
+  Box box;
+  int ntx, nty, nkc, chunk;
+};
+inline TileMarch tile_march(const Box& b, int chunk) {
+  TileMarch t;
+  t.box = b;
+  t.ntx = (b.n[0] + BOX_X - 1) / BOX_X;
+  t.nty = (b.n[1] + BOX_Y - 1) / BOX_Y;
+  t.nkc = (b.n[2] + chunk - 1) / chunk;
+  t.chunk = chunk;
+  return t;
+}
+inline unsigned tile_march_blocks(const TileMarch& t) {
+  const unsigned nb = (unsigned)t.ntx * t.nty * t.nkc;
+  return (nb + N_XCD - 1) / N_XCD * N_XCD;
+}
+__device__ inline bool tile_march_node(const TileMarch& t, int& i, int& j, int& k0, int& k1) {
+  const unsigned nb = (unsigned)t.ntx * t.nty * t.nkc;
+  const unsigned per = (nb + N_XCD - 1) / N_XCD;
+  const unsigned lb = (blockIdx.x % N_XCD) * per + blockIdx.x / N_XCD;
+  if (lb >= nb) return false;
+  const unsigned tx = lb % t.ntx, r = lb / t.ntx;
+  const unsigned ty = r % t.nty, kc = r / t.nty;
+  const int ti = tx * BOX_X + threadIdx.x, tj = ty * BOX_Y + threadIdx.y;
+  if (ti >= t.box.n[0] || tj >= t.box.n[1]) return false;
+  i = t.box.lo[0] + ti;
+  j = t.box.lo[1] + tj;
+  k0 = t.box.lo[2] + (int)kc * t.chunk;
+  k1 = min(k0 + t.chunk, t.box.lo[2] + t.box.n[2]);
+  return true;
 }
 
 }  // namespace cbns
@@ -136,6 +200,7 @@ int launch_metrics(cbns_block* b, const double* dx, const double* dy, const doub
 int launch_gradient_nodes(hipStream_t s, int IM, int JM, int KM, const double* f, double* fx,
                           double* fy, double* fz);
 int launch_bc_primitive(cbns_block* b, double* Q);
+int launch_bc_only(cbns_block* b, double* Q);
 int launch_primitive_all(cbns_block* b, const double* Q);
 int launch_conservative_all(cbns_block* b, double* Q);
 int launch_freestream(cbns_block* b);

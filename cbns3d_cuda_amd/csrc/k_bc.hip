@@ -40,7 +40,7 @@ __device__ inline void copy_q(const Consts& c, double* __restrict__ Q, int d, in
 __global__ void k_prim_box(Consts c, Box box, const double* __restrict__ Q,
                            double* __restrict__ prim) {
   int i, j, k;
-  if (!box_node(box, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  if (!box_node3(box, i, j, k)) return;
   const int o = idx(c, i, j, k);
   prim_from_q(c, Q, prim, o, o);
 }
@@ -222,6 +222,48 @@ __global__ void k_wrap(Consts c, int a, double* __restrict__ Q, double* __restri
   for (int q = 0; q < nprim; q++) prim[(long)q * c.S + d] = prim[(long)q * c.S + s];
 }
 
+// All axes periodic: every ghost node copies the node at its coordinates
+// wrapped on all three axes at once (the same values the axis-by-axis wraps
+// produce).  The ghost shell is enumerated slab by slab: zeta slabs (full
+// extent), then eta slabs (zeta in range), then xi slabs (eta, zeta in range).
+__global__ void k_wrap_shell(Consts c, double* __restrict__ Q, double* __restrict__ prim,
+                             int nprim) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  const unsigned PX = c.PX, PY = c.PY, twoG = 2 * G;
+  const unsigned nz = twoG * PX * PY, ny = twoG * PX * (unsigned)c.KM,
+                 nx = twoG * (unsigned)c.JM * (unsigned)c.KM;
+  int i, j, k;
+  if (t < nz) {
+    const unsigned g = t / (PX * PY), r = t - g * PX * PY;
+    k = g < (unsigned)G ? (int)g - G : c.KM + (int)g - G;
+    j = (int)(r / PX) - G;
+    i = (int)(r - (r / PX) * PX) - G;
+  } else if (t < nz + ny) {
+    const unsigned u = t - nz;
+    const unsigned kk = u / (twoG * PX), r = u - kk * twoG * PX;
+    const unsigned g = r / PX;
+    k = (int)kk;
+    j = g < (unsigned)G ? (int)g - G : c.JM + (int)g - G;
+    i = (int)(r - g * PX) - G;
+  } else if (t < nz + ny + nx) {
+    const unsigned u = t - nz - ny;
+    const unsigned kk = u / (twoG * (unsigned)c.JM), r = u - kk * twoG * (unsigned)c.JM;
+    const unsigned jj = r / twoG, g = r - jj * twoG;
+    k = (int)kk;
+    j = (int)jj;
+    i = g < (unsigned)G ? (int)g - G : c.IM + (int)g - G;
+  } else {
+    return;
+  }
+  const int si = i < 0 ? i + c.IM : (i >= c.IM ? i - c.IM : i);
+  const int sj = j < 0 ? j + c.JM : (j >= c.JM ? j - c.JM : j);
+  const int sk = k < 0 ? k + c.KM : (k >= c.KM ? k - c.KM : k);
+  const int d = idx(c, i, j, k), s = idx(c, si, sj, sk);
+#pragma unroll
+  for (int q = 0; q < NEQ; q++) Q[(long)q * c.S + d] = Q[(long)q * c.S + s];
+  for (int q = 0; q < nprim; q++) prim[(long)q * c.S + d] = prim[(long)q * c.S + s];
+}
+
 template <class K, class... A>
 int run(cbns_block* b, const Box& box, K kern, A... args) {
   if (box.count() <= 0) return CBNS_OK;
@@ -232,16 +274,13 @@ int run(cbns_block* b, const Box& box, K kern, A... args) {
 
 }  // namespace
 
-int launch_bc_primitive(cbns_block* b, double* Q) {
+// Boundary rules and ghost fill for state Q whose interior primitives are
+// already in b->prim (the fused stage writes them with the new state).
+int launch_bc_only(cbns_block* b, double* Q) {
   const Consts& c = b->c;
   int rc;
-  const int ilo = upd_lo(c, 0), ihi = upd_hi(c, 0);
   const int jlo = upd_lo(c, 1), jhi = upd_hi(c, 1);
   const int klo = upd_lo(c, 2), khi = upd_hi(c, 2);
-  {
-    Box box{{ilo, jlo, klo}, {ihi - ilo + 1, jhi - jlo + 1, khi - klo + 1}};
-    if ((rc = run(b, box, k_prim_box, (const double*)Q, b->prim))) return rc;
-  }
   if (!periodic(c, 0)) {
     Box box{{0, jlo, klo}, {2, jhi - jlo + 1, khi - klo + 1}};
     if ((rc = run(b, box, k_bc_xi, Q, b->prim))) return rc;
@@ -259,6 +298,12 @@ int launch_bc_primitive(cbns_block* b, double* Q) {
     if ((rc = run(b, box, k_bc_zeta, Q, b->prim))) return rc;
   }
   const int nprim = c.viscous ? 6 : 5;
+  if (periodic(c, 0) && periodic(c, 1) && periodic(c, 2)) {
+    const long cnt = (long)c.S - (long)c.IM * c.JM * c.KM;
+    k_wrap_shell<<<(unsigned)((cnt + 255) / 256), 256, 0, b->stream>>>(c, Q, b->prim, nprim);
+    CBNS_HIP(b, hipGetLastError());
+    return CBNS_OK;
+  }
   const int P[3] = {c.PX, c.PY, c.PZ};
   for (int a = 0; a < 3; a++) {
     if (!periodic(c, a)) continue;
@@ -267,6 +312,18 @@ int launch_bc_primitive(cbns_block* b, double* Q) {
     CBNS_HIP(b, hipGetLastError());
   }
   return CBNS_OK;
+}
+
+int launch_bc_primitive(cbns_block* b, double* Q) {
+  const Consts& c = b->c;
+  Box box;
+  for (int a = 0; a < 3; a++) {
+    box.lo[a] = upd_lo(c, a);
+    box.n[a] = upd_hi(c, a) - upd_lo(c, a) + 1;
+  }
+  k_prim_box<<<box_grid(box), box_block(), 0, b->stream>>>(c, box, Q, b->prim);
+  CBNS_HIP(b, hipGetLastError());
+  return launch_bc_only(b, Q);
 }
 
 }  // namespace cbns

@@ -10,7 +10,8 @@
 //                from rhs (k_viscous.hip)
 //   k_march<2>   the same march along zeta; adds dG, divides by J and
 //                applies the Shu-Osher stage combination, writing the stage
-//                output.
+//                output and its primitives (the boundary pass then only
+//                fills boundaries and ghosts).
 //
 // Node primitives for the Roe average are recomputed from Q (the
 // primitive array holds exactly f(Q) at every node a face reads:
@@ -45,7 +46,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE_X,
                                                 const double* __restrict__ met,
                                                 double* __restrict__ fx) {
   int i, j, k;
-  if (!box_node(box, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  if (!box_node3(box, i, j, k)) return;
   const long S = c.S;
   const int a = idx(c, i, j, k), b = a + 1;
   const double Ja = met[JAC * S + a], Jb = met[JAC * S + b];
@@ -66,6 +67,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE_X,
 struct StageArgs {
   double dt;
   int stage;
+  double* prim;         // primitives of the stage output (AX == 2)
   const double* Qn;     // level-n state (stages 2, 3)
   const double* Qin;    // stage input (stencil source)
   double* Qout;         // stage output
@@ -137,6 +139,7 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
       }
     } else {
       const double inv_j = 1.0 / J[o];
+      double qn_out[5];
 #pragma unroll
       for (int n = 0; n < 5; n++) {
         double df = sa.rhs[n * S + o] + (f[n] - fprev[n]);
@@ -148,6 +151,18 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
         else if (sa.stage == 2) out = 0.75 * sa.Qn[n * S + o] + 0.25 * (qin - sa.dt * df);
         else out = (sa.Qn[n * S + o] + 2.0 * (qin - sa.dt * df)) * (1.0 / 3.0);
         sa.Qout[n * S + o] = out;
+        qn_out[n] = out;
+      }
+      // primitives of the new state (src/primitive_bc_kernel.cu:63-76)
+      {
+        double rr, uu, vv, ww, pp;
+        q_to_prim(g, qn_out, rr, uu, vv, ww, pp);
+        sa.prim[o] = rr;
+        sa.prim[S + o] = uu;
+        sa.prim[2 * S + o] = vv;
+        sa.prim[3 * S + o] = ww;
+        sa.prim[4 * S + o] = pp;
+        if (c.viscous) sa.prim[5 * S + o] = c.gM2 * pp * fast_rcp(rr);
       }
     }
 #pragma unroll
@@ -169,10 +184,10 @@ int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, d
     box.lo[0] = periodic(c, 0) ? -1 : 0;
     box.n[0] = c.IM - 1 + (periodic(c, 0) ? 2 : 0);  // faces -1..N-1 or 0..N-2
     ProfScope ps(b, CBNS_K_XFACES);
-    k_xfaces<<<box_blocks(box, 256), 256, 0, b->stream>>>(c, box, qin, b->met, fx);
+    k_xfaces<<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fx);
     CBNS_HIP(b, hipGetLastError());
   }
-  StageArgs sa{dt, stage, b->Q, qin, qout, fx, b->rhs, nullptr};
+  StageArgs sa{dt, stage, b->prim, b->Q, qin, qout, fx, b->rhs, nullptr};
   const int ni = upd_hi(c, 0) - upd_lo(c, 0) + 1;
   {
     const int nt = upd_hi(c, 2) - upd_lo(c, 2) + 1;
@@ -184,7 +199,6 @@ int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, d
     CBNS_HIP(b, hipGetLastError());
   }
   if (c.viscous) {
-    ProfScope ps(b, CBNS_K_VISCOUS);
     int rc = launch_viscous(b, b->vtmp, nullptr, b->rhs, 1);
     if (rc) return rc;
   }

@@ -18,7 +18,7 @@ __global__ void __launch_bounds__(256) k_face_flux(Consts c, Box box, int d,
                                                    const double* __restrict__ met,
                                                    double* __restrict__ flux) {
   int i, j, k;
-  if (!box_node(box, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  if (!box_node3(box, i, j, k)) return;
   const long S = c.S;
   const int st = stride(c, d);
   const int a = idx(c, i, j, k), bb = a + st;
@@ -47,7 +47,7 @@ __global__ void k_update_rk3(Consts c, Box box, double dt, int stage, const doub
                              const double* __restrict__ flux, const double* __restrict__ dfv,
                              double* __restrict__ Q, double* __restrict__ Qp) {
   int i, j, k;
-  if (!box_node(box, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+  if (!box_node3(box, i, j, k)) return;
   const long S = c.S;
   const int o = idx(c, i, j, k);
   const double Jac_a = met[JAC * S + o];
@@ -84,7 +84,7 @@ int launch_numerical_flux(cbns_block* b, const double* Q) {
     Box box = update_box(c);
     box.lo[d] = periodic(c, d) ? -1 : 0;
     box.n[d] = axis_n(c, d) - 1 + (periodic(c, d) ? 2 : 0);  // faces -1..N-1 or 0..N-2
-    k_face_flux<<<box_blocks(box, 256), 256, 0, b->stream>>>(c, box, d, Q, b->prim, b->met,
+    k_face_flux<<<box_grid(box), box_block(), 0, b->stream>>>(c, box, d, Q, b->prim, b->met,
                                                              b->flux);
     CBNS_HIP(b, hipGetLastError());
   }
@@ -97,7 +97,7 @@ int launch_viscous_unfused(cbns_block* b) {
 
 int launch_update_rk3(cbns_block* b, double dt, int stage) {
   const Box ub = update_box(b->c);
-  k_update_rk3<<<box_blocks(ub, 256), 256, 0, b->stream>>>(b->c, ub, dt, stage, b->met, b->flux,
+  k_update_rk3<<<box_grid(ub), box_block(), 0, b->stream>>>(b->c, ub, dt, stage, b->met, b->flux,
                                                            b->dfv, b->Q, b->qp);
   CBNS_HIP(b, hipGetLastError());
   return CBNS_OK;

@@ -12,11 +12,17 @@
 //                  nodes, at the reference's transverse offset (-1) or
 //                  centred; written to dfv (reference decomposition) or
 //                  subtracted from the fused stage's right-hand side.
+// Both walk along k in chunks on XCD-contiguous tiles (TileMarch) so the
+// +-2 row and plane neighbours of the stencils are re-read from L2, not HBM.
 #include "cbns_internal.hpp"
 
 namespace cbns {
 
 namespace {
+
+#ifndef VISC_CHUNK
+#define VISC_CHUNK 16  // nodes along k per thread
+#endif
 
 // 4th-order derivative along axis a at node m of the viscous range:
 // one-sided at the two outermost layers of a reference axis
@@ -42,13 +48,11 @@ __device__ __forceinline__ double dgrad(const Consts& c, int a, int m, const dou
          1.0 / 12.0 * f[2 * st];
 }
 
-__global__ void __launch_bounds__(256) k_visc_fluxes(Consts c, Box box,
-                                                     const double* __restrict__ prim,
-                                                     const double* __restrict__ met,
-                                                     double* __restrict__ mu,
-                                                     double* __restrict__ vf) {
-  int i, j, k;
-  if (!box_node(box, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, int k,
+                                                 const double* __restrict__ prim,
+                                                 const double* __restrict__ met,
+                                                 double* __restrict__ mu,
+                                                 double* __restrict__ vf) {
   const long S = c.S;
   const int o = idx(c, i, j, k);
   const double* U = prim + P_U * S + o;
@@ -102,12 +106,21 @@ __global__ void __launch_bounds__(256) k_visc_fluxes(Consts c, Box box,
   }
 }
 
+__global__ void __launch_bounds__(256) k_visc_fluxes(Consts c, TileMarch tm,
+                                                     const double* __restrict__ prim,
+                                                     const double* __restrict__ met,
+                                                     double* __restrict__ mu,
+                                                     double* __restrict__ vf) {
+  int i, j, k0, k1;
+  if (!tile_march_node(tm, i, j, k0, k1)) return;
+  for (int k = k0; k < k1; k++) visc_fluxes_node(c, i, j, k, prim, met, mu, vf);
+}
+
 // src/viscous_contribution_kernel.cu:35-67.  mode 0: dfv[e] = div; mode 1:
 // out[(e+1)] -= div (the fused stage's right-hand side, momentum and energy).
-__global__ void __launch_bounds__(256) k_visc_div(Consts c, Box box, const double* __restrict__ vf,
-                                                  double* __restrict__ out, int mode) {
-  int i, j, k;
-  if (!box_node(box, (long)blockIdx.x * blockDim.x + threadIdx.x, i, j, k)) return;
+__device__ __forceinline__ void visc_div_node(const Consts& c, int i, int j, int k,
+                                              const double* __restrict__ vf,
+                                              double* __restrict__ out, int mode) {
   const long S = c.S;
   const int sh = c.visc_centered ? 0 : -1;
   const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
@@ -128,6 +141,14 @@ __global__ void __launch_bounds__(256) k_visc_div(Consts c, Box box, const doubl
   }
 }
 
+__global__ void __launch_bounds__(256) k_visc_div(Consts c, TileMarch tm,
+                                                  const double* __restrict__ vf,
+                                                  double* __restrict__ out, int mode) {
+  int i, j, k0, k1;
+  if (!tile_march_node(tm, i, j, k0, k1)) return;
+  for (int k = k0; k < k1; k++) visc_div_node(c, i, j, k, vf, out, mode);
+}
+
 }  // namespace
 
 // vf: 12*S scratch; mu may be null.  mode 0 writes dfv (4*S) to `out`,
@@ -139,14 +160,21 @@ int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode)
     vb.lo[a] = -2;
     vb.n[a] = axis_n(c, a) + 4;
   }
-  k_visc_fluxes<<<box_blocks(vb, 256), 256, 0, b->stream>>>(c, vb, b->prim, b->met, mu, vf);
-  CBNS_HIP(b, hipGetLastError());
+  {
+    ProfScope ps(b, CBNS_K_VISC_FLUX);
+    const TileMarch tm = tile_march(vb, VISC_CHUNK);
+    k_visc_fluxes<<<tile_march_blocks(tm), box_block(), 0, b->stream>>>(c, tm, b->prim, b->met,
+                                                                       mu, vf);
+    CBNS_HIP(b, hipGetLastError());
+  }
   Box ub;
   for (int a = 0; a < 3; a++) {
     ub.lo[a] = upd_lo(c, a);
     ub.n[a] = upd_hi(c, a) - upd_lo(c, a) + 1;
   }
-  k_visc_div<<<box_blocks(ub, 256), 256, 0, b->stream>>>(c, ub, vf, out, mode);
+  ProfScope ps(b, CBNS_K_VISC_DIV);
+  const TileMarch tm = tile_march(ub, VISC_CHUNK);
+  k_visc_div<<<tile_march_blocks(tm), box_block(), 0, b->stream>>>(c, tm, vf, out, mode);
   CBNS_HIP(b, hipGetLastError());
   return CBNS_OK;
 }

@@ -153,10 +153,11 @@ enum {
   CBNS_K_XFACES = 0,     /* xi-face fluxes                          */
   CBNS_K_MARCH_ETA = 1,  /* eta sweep (+ xi difference)             */
   CBNS_K_MARCH_ZETA = 2, /* zeta sweep + RK stage combination       */
-  CBNS_K_VISCOUS = 3,    /* viscous terms and divergence            */
-  CBNS_K_BC = 4,         /* boundary / ghost fill + primitives      */
-  CBNS_K_DT = 5,         /* CFL time step reduction                 */
-  CBNS_K_COUNT = 6
+  CBNS_K_VISC_FLUX = 3,  /* viscous fluxes Ev,Fv,Gv at the nodes     */
+  CBNS_K_VISC_DIV = 4,   /* viscous divergence into the stage rhs   */
+  CBNS_K_BC = 5,         /* boundary / ghost fill                   */
+  CBNS_K_DT = 6,         /* CFL time step reduction                 */
+  CBNS_K_COUNT = 7
 };
 int cbns_set_profiling(cbns_block* blk, int32_t on);
 int cbns_kernel_time(cbns_block* blk, int32_t kernel, int32_t reset, double* ms, int64_t* launches);

@@ -96,192 +96,9 @@ static int periodic(const orc_grid* g, int a) { return g->bc[a] == ORC_BC_PERIOD
 static int upd_lo(const orc_grid* g, int a) { return periodic(g, a) ? 0 : 1; }
 static int upd_hi(const orc_grid* g, int a) { return periodic(g, a) ? axis_n(g, a) - 1 : axis_n(g, a) - 2; }
 
-/* ------------------------------------------------------------------------ */
-/* src/weno.cu:13-44 */
-double orc_weno5(double Um2, double Um1, double U0, double Up1, double Up2) {
-  const double eps = 1.0e-6;
-  double diff1 = Um2 - 2.0 * Um1 + U0;
-  double diff2 = Um2 - 4.0 * Um1 + 3.0 * U0;
-  double diff3 = Um1 - 2.0 * U0 + Up1;
-  double diff4 = Um1 - Up1;
-  double diff5 = U0 - 2.0 * Up1 + Up2;
-  double diff6 = 3.0 * U0 - 4.0 * Up1 + Up2;
-  double IS1 = (13.0 / 12.0) * diff1 * diff1 + 0.25 * diff2 * diff2 + eps;
-  double IS2 = (13.0 / 12.0) * diff3 * diff3 + 0.25 * diff4 * diff4 + eps;
-  double IS3 = (13.0 / 12.0) * diff5 * diff5 + 0.25 * diff6 * diff6 + eps;
-  double w1 = 0.1 / (IS1 * IS1);
-  double w2 = 0.6 / (IS2 * IS2);
-  double w3 = 0.3 / (IS3 * IS3);
-  double sum = w1 + w2 + w3;
-  double f1 = (2.0 * Um2 - 7.0 * Um1 + 11.0 * U0) / 6.0;
-  double f2 = (-Um1 + 5.0 * U0 + 2.0 * Up1) / 6.0;
-  double f3 = (2.0 * U0 + 5.0 * Up1 - Up2) / 6.0;
-  return (w1 * f1 + w2 * f2 + w3 * f3) / sum;
-}
-
-/* src/calc_eigenvectors_roe.cu:13-152.  pl/pr = (rho,u,v,w,p); R, L row-major */
-void orc_eigenvectors_roe(const orc_grid* g, const double pl[5], const double pr[5],
-                          double n_x, double n_y, double n_z, double R[25], double L[25]) {
-  const double gamma = g->gamma, gam1 = g->gam1;
-  const double rho_L = pl[0], u_L = pl[1], v_L = pl[2], w_L = pl[3], p_L = pl[4];
-  const double rho_R = pr[0], u_R = pr[1], v_R = pr[2], w_R = pr[3], p_R = pr[4];
-  double l_x = 0.0, l_y = n_z, l_z = -n_y;
-  double m_x, m_y, m_z;
-  double n_abs = sqrt(l_x * l_x + l_y * l_y + l_z * l_z);
-  if (n_abs > 1.0e-8) {
-    m_x = -n_y * n_y - n_z * n_z;
-    m_y = n_x * n_y;
-    m_z = n_x * n_z;
-  } else {
-    l_x = -n_z; l_y = 0.0; l_z = n_x;
-    m_x = n_x * n_y;
-    m_y = -n_x * n_x - n_z * n_z;
-    m_z = n_y * n_z;
-  }
-  n_abs = sqrt(n_x * n_x + n_y * n_y + n_z * n_z);
-  n_x /= n_abs; n_y /= n_abs; n_z /= n_abs;
-  n_abs = sqrt(l_x * l_x + l_y * l_y + l_z * l_z);
-  l_x /= n_abs; l_y /= n_abs; l_z /= n_abs;
-  n_abs = sqrt(m_x * m_x + m_y * m_y + m_z * m_z);
-  m_x /= n_abs; m_y /= n_abs; m_z /= n_abs;
-
-  double c_L = sqrt(gamma * p_L / rho_L);
-  double H_L = c_L * c_L / gam1 + 0.5 * (u_L * u_L + v_L * v_L + w_L * w_L);
-  double c_R = sqrt(gamma * p_R / rho_R);
-  double H_R = c_R * c_R / gam1 + 0.5 * (u_R * u_R + v_R * v_R + w_R * w_R);
-  double ratio = sqrt(rho_R / rho_L);
-  double uu = (u_L + ratio * u_R) / (1.0 + ratio);
-  double vv = (v_L + ratio * v_R) / (1.0 + ratio);
-  double ww = (w_L + ratio * w_R) / (1.0 + ratio);
-  double H = (H_L + ratio * H_R) / (1.0 + ratio);
-  double q2 = uu * uu + vv * vv + ww * ww;
-  double c = sqrt(gam1 * (H - 0.5 * q2));
-  double qn = uu * n_x + vv * n_y + ww * n_z;
-  double ql = uu * l_x + vv * l_y + ww * l_z;
-  double qm = uu * m_x + vv * m_y + ww * m_z;
-
-#define RR(r, cc) R[(r) * 5 + (cc)]
-#define LL(r, cc) L[(r) * 5 + (cc)]
-  RR(0, 0) = 1.0; RR(1, 0) = uu - c * n_x; RR(2, 0) = vv - c * n_y; RR(3, 0) = ww - c * n_z; RR(4, 0) = H - qn * c;
-  RR(0, 1) = 1.0; RR(1, 1) = uu;           RR(2, 1) = vv;           RR(3, 1) = ww;           RR(4, 1) = 0.5 * q2;
-  RR(0, 2) = 1.0; RR(1, 2) = uu + c * n_x; RR(2, 2) = vv + c * n_y; RR(3, 2) = ww + c * n_z; RR(4, 2) = H + qn * c;
-  RR(0, 3) = 0.0; RR(1, 3) = l_x;          RR(2, 3) = l_y;          RR(3, 3) = l_z;          RR(4, 3) = ql;
-  RR(0, 4) = 0.0; RR(1, 4) = m_x;          RR(2, 4) = m_y;          RR(3, 4) = m_z;          RR(4, 4) = qm;
-
-  double K_2c2 = gam1 / (2.0 * c * c);
-  double one_2c = 0.5 / c;
-  LL(0, 0) = 0.5 * K_2c2 * q2 + one_2c * qn;
-  LL(1, 0) = 1.0 - K_2c2 * q2;
-  LL(2, 0) = 0.5 * K_2c2 * q2 - one_2c * qn;
-  LL(3, 0) = -ql;
-  LL(4, 0) = -qm;
-  LL(0, 1) = -(K_2c2 * uu + one_2c * n_x);
-  LL(1, 1) = 2.0 * K_2c2 * uu;
-  LL(2, 1) = -(K_2c2 * uu - one_2c * n_x);
-  LL(3, 1) = l_x;
-  LL(4, 1) = m_x;
-  LL(0, 2) = -(K_2c2 * vv + one_2c * n_y);
-  LL(1, 2) = 2.0 * K_2c2 * vv;
-  LL(2, 2) = -(K_2c2 * vv - one_2c * n_y);
-  LL(3, 2) = l_y;
-  LL(4, 2) = m_y;
-  LL(0, 3) = -(K_2c2 * ww + one_2c * n_z);
-  LL(1, 3) = 2.0 * K_2c2 * ww;
-  LL(2, 3) = -(K_2c2 * ww - one_2c * n_z);
-  LL(3, 3) = l_z;
-  LL(4, 3) = m_z;
-  LL(0, 4) = K_2c2;
-  LL(1, 4) = -2.0 * K_2c2;
-  LL(2, 4) = K_2c2;
-  LL(3, 4) = 0.0;
-  LL(4, 4) = 0.0;
-#undef RR
-#undef LL
-}
-
-/* src/weno.cu:46-120.  f = 5 stencil points x 5 conservative components
- * (f[p*5+c]); output prim = (rho,u,v,w,p) of the reconstructed state. */
-void orc_rec_weno5(const orc_grid* g, const double f[25], const double R[25],
-                   const double L[25], double prim[5]) {
-  double f_sten[NEQ][5], fc_sten[NEQ][5];
-  for (int pnt = 0; pnt < 5; pnt++)
-    for (int c = 0; c < NEQ; c++) f_sten[c][pnt] = f[pnt * 5 + c];
-  for (int li = 0; li < NEQ; li++)
-    for (int lj = 0; lj < 5; lj++) {
-      double ss = 0.0;
-      for (int lk = 0; lk < NEQ; lk++) ss += L[li * 5 + lk] * f_sten[lk][lj];
-      fc_sten[li][lj] = ss;
-    }
-  for (int li = 0; li < NEQ; li++)
-    f_sten[li][0] = orc_weno5(fc_sten[li][0], fc_sten[li][1], fc_sten[li][2],
-                              fc_sten[li][3], fc_sten[li][4]);
-  for (int li = 0; li < NEQ; li++) {
-    double ss = 0.0;
-    for (int lj = 0; lj < NEQ; lj++) ss += R[li * 5 + lj] * f_sten[lj][0];
-    f_sten[li][1] = ss;
-  }
-  double rho = f_sten[0][1];
-  double u = f_sten[1][1] / rho;
-  double v = f_sten[2][1] / rho;
-  double w = f_sten[3][1] / rho;
-  double p = g->gam1 * (f_sten[4][1] - 0.5 * rho * (u * u + v * v + w * w));
-  prim[0] = rho; prim[1] = u; prim[2] = v; prim[3] = w; prim[4] = p;
-}
-
-/* src/lf_flux.cu:13-80 (note the argument order: right state first) */
-void orc_lf_flux(const orc_grid* g, const double pr[5], const double pl[5],
-                 double nx, double ny, double nz, double flux[5]) {
-  const double gamma = g->gamma, gam1 = g->gam1;
-  const double rho_R = pr[0], u_R = pr[1], v_R = pr[2], w_R = pr[3], p_R = pr[4];
-  const double rho_L = pl[0], u_L = pl[1], v_L = pl[2], w_L = pl[3], p_L = pl[4];
-  double c_R = sqrt(gamma * p_R / rho_R);
-  double H_R = c_R * c_R / gam1 + 0.5 * (u_R * u_R + v_R * v_R + w_R * w_R);
-  double rE_R = p_R / gam1 + 0.5 * rho_R * (u_R * u_R + v_R * v_R + w_R * w_R);
-  double c_L = sqrt(gamma * p_L / rho_L);
-  double H_L = c_L * c_L / gam1 + 0.5 * (u_L * u_L + v_L * v_L + w_L * w_L);
-  double rE_L = p_L / gam1 + 0.5 * rho_L * (u_L * u_L + v_L * v_L + w_L * w_L);
-  double q_R = u_R * nx + v_R * ny + w_R * nz;
-  double q_L = u_L * nx + v_L * ny + w_L * nz;
-  double ws[3], ws_max;
-  ws[0] = fabs(u_R) + c_R; ws_max = fabs(u_L) + c_L; if (ws_max > ws[0]) ws[0] = ws_max;
-  ws[1] = fabs(v_R) + c_R; ws_max = fabs(v_L) + c_L; if (ws_max > ws[1]) ws[1] = ws_max;
-  ws[2] = fabs(w_R) + c_R; ws_max = fabs(w_L) + c_L; if (ws_max > ws[2]) ws[2] = ws_max;
-  ws_max = fabs(ws[0] * nx + ws[1] * ny + ws[2] * nz);
-  double diss[NEQ], f_L[NEQ], f_R[NEQ];
-  diss[0] = ws_max * (rho_R - rho_L);
-  diss[1] = ws_max * (rho_R * u_R - rho_L * u_L);
-  diss[2] = ws_max * (rho_R * v_R - rho_L * v_L);
-  diss[3] = ws_max * (rho_R * w_R - rho_L * w_L);
-  diss[4] = ws_max * (rE_R - rE_L);
-  f_L[0] = rho_L * q_L;
-  f_L[1] = rho_L * q_L * u_L + p_L * nx;
-  f_L[2] = rho_L * q_L * v_L + p_L * ny;
-  f_L[3] = rho_L * q_L * w_L + p_L * nz;
-  f_L[4] = rho_L * q_L * H_L;
-  f_R[0] = rho_R * q_R;
-  f_R[1] = rho_R * q_R * u_R + p_R * nx;
-  f_R[2] = rho_R * q_R * v_R + p_R * ny;
-  f_R[3] = rho_R * q_R * w_R + p_R * nz;
-  f_R[4] = rho_R * q_R * H_R;
-  for (int li = 0; li < NEQ; li++) flux[li] = 0.5 * (f_L[li] + f_R[li] - diss[li]);
-}
-
-/* One face of src/rec_riemann_kernel.cu:44-89: q holds the 6 nodes m-2..m+3
- * along the sweep (q[s*5+c]); pl/pr are the node primitives at m and m+1;
- * (nx,ny,nz) the averaged unnormalised face vector. */
-void orc_face_flux(const orc_grid* g, const double q[30], const double pl[5],
-                   const double pr[5], double nx, double ny, double nz, double flux[5]) {
-  double R[25], L[25], sl[25], sr[25], wl[5], wr[5];
-  orc_eigenvectors_roe(g, pl, pr, nx, ny, nz, R, L);
-  for (int p = 0; p < 5; p++)
-    for (int c = 0; c < 5; c++) {
-      sl[p * 5 + c] = q[p * 5 + c];             /* m-2 .. m+2 */
-      sr[p * 5 + c] = q[(5 - p) * 5 + c];       /* m+3 .. m-1 */
-    }
-  orc_rec_weno5(g, sl, R, L, wl);
-  orc_rec_weno5(g, sr, R, L, wr);
-  orc_lf_flux(g, wr, wl, nx, ny, nz, flux);
-}
+#define ORC_REAL double
+#define ORC_FN(name) orc_##name
+#include "orc_point.inc"
 
 /* ------------------------------------------------------------------------ */
 /* gradient stencils, src/gradient_kernel.cu:40-131 (one axis).  fm(o) returns

@@ -219,3 +219,19 @@ def test_face_flux_upwinds_a_step(oracle):
     a = max(cl, cr)
     ref = 0.5 * (euler_flux_n(ql, (1, 0, 0)) + euler_flux_n(qr, (1, 0, 0))) - 0.5 * a * (qr - ql)
     assert np.allclose(f, ref, rtol=1e-8, atol=1e-8)
+
+
+def test_face_operation_count():
+    """The reference algorithm's floating-point work per face, counted on the
+    oracle with an operation-counting scalar (oracle/opcount.cpp); bench.py
+    divides this count by the measured face-kernel time."""
+    import json
+    import os
+    import subprocess
+    here = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle")
+    subprocess.run(["make", "-s", "-C", here, "opcount"], check=True)
+    out = subprocess.run([os.path.join(here, "opcount")], capture_output=True, text=True, check=True)
+    rows = [json.loads(l) for l in out.stdout.splitlines()]
+    for r in rows:
+        assert (r["add_sub"], r["mul"], r["div"], r["sqrt"]) == (673, 795, 102, 10)
+        assert r["flops"] == 1580
