@@ -207,6 +207,7 @@ int launch_freestream(cbns_block* b);
 int launch_numerical_flux(cbns_block* b, const double* Q);
 int launch_viscous_unfused(cbns_block* b);
 int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode);
+int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs);
 int launch_update_rk3(cbns_block* b, double dt, int stage);
 int launch_calc_dt(cbns_block* b, double* dt_host);
 int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout);

@@ -279,4 +279,88 @@ __device__ __forceinline__ void face_flux(const GasT<T>& g, const T q[6][5], con
   lf_flux(g, rR, uR, vR, wR, pR, rL, uL, vL, wL, pL, nx, ny, nz, f);
 }
 
+// Precision form of face_flux: the Roe basis, the characteristic projection
+// and WENO5 run in T (float) on the deviations of the six stencil nodes from
+// node m (WENO5-JS is translation invariant: weno5(x + c) = weno5(x) + c, and
+// the projection is linear), so T's rounding acts on the local variation, not
+// on the state itself.  The reconstructed states q_m + dq_L/R and the
+// Lax-Friedrichs flux are formed in double, so face-flux differences carry no
+// float cancellation error.
+template <class T>
+__device__ __forceinline__ void face_flux_mixed(const GasT<T>& gt, const GasT<double>& g,
+                                                const double q[6][5], const double pl[5],
+                                                const double pr[5], double nx, double ny,
+                                                double nz, double f[5]) {
+  Basis<T> B;
+  roe_basis(gt, (T)pl[0], (T)pl[1], (T)pl[2], (T)pl[3], (T)pl[4], (T)pr[0], (T)pr[1], (T)pr[2],
+            (T)pr[3], (T)pr[4], (T)nx, (T)ny, (T)nz, B);
+  T d[6][5];
+#pragma unroll
+  for (int s = 0; s < 6; s++)
+#pragma unroll
+    for (int c = 0; c < 5; c++) d[s][c] = (T)(q[s][c] - q[2][c]);
+  T A[6], Bn[6];
+#pragma unroll
+  for (int s = 0; s < 6; s++) {
+    A[s] = B.K * (d[s][4] - (B.uu * d[s][1] + B.vv * d[s][2] + B.ww * d[s][3]) +
+                  T(0.5) * B.q2 * d[s][0]);
+    Bn[s] = B.o * (B.qn * d[s][0] - (B.nx * d[s][1] + B.ny * d[s][2] + B.nz * d[s][3]));
+  }
+  T WL[5], WR[5];
+  weno5_pair(A[0] + Bn[0], A[1] + Bn[1], A[2] + Bn[2], A[3] + Bn[3], A[4] + Bn[4], A[5] + Bn[5],
+             WL[0], WR[0]);
+  weno5_pair(d[0][0] - T(2) * A[0], d[1][0] - T(2) * A[1], d[2][0] - T(2) * A[2],
+             d[3][0] - T(2) * A[3], d[4][0] - T(2) * A[4], d[5][0] - T(2) * A[5], WL[1], WR[1]);
+  weno5_pair(A[0] - Bn[0], A[1] - Bn[1], A[2] - Bn[2], A[3] - Bn[3], A[4] - Bn[4], A[5] - Bn[5],
+             WL[2], WR[2]);
+  T w[6];
+#pragma unroll
+  for (int s = 0; s < 6; s++)
+    w[s] = -B.ql * d[s][0] + B.lx * d[s][1] + B.ly * d[s][2] + B.lz * d[s][3];
+  weno5_pair(w[0], w[1], w[2], w[3], w[4], w[5], WL[3], WR[3]);
+#pragma unroll
+  for (int s = 0; s < 6; s++)
+    w[s] = -B.qm * d[s][0] + B.mx * d[s][1] + B.my * d[s][2] + B.mz * d[s][3];
+  weno5_pair(w[0], w[1], w[2], w[3], w[4], w[5], WL[4], WR[4]);
+  // dq = R W (deviation), then q = q_m + dq in double
+  double qL[5], qR[5];
+  {
+    const T* Ws[2] = {WL, WR};
+    double* out[2] = {qL, qR};
+#pragma unroll
+    for (int side = 0; side < 2; side++) {
+      const T* W = Ws[side];
+      const T sm = W[0] + W[2];
+      const T dl = B.c * (W[2] - W[0]);
+      const T rho = sm + W[1];
+      const T m1 = B.uu * rho + B.nx * dl + B.lx * W[3] + B.mx * W[4];
+      const T m2 = B.vv * rho + B.ny * dl + B.ly * W[3] + B.my * W[4];
+      const T m3 = B.ww * rho + B.nz * dl + B.lz * W[3] + B.mz * W[4];
+      const T E = B.H * sm + T(0.5) * B.q2 * W[1] + B.qn * dl + B.ql * W[3] + B.qm * W[4];
+      out[side][0] = q[2][0] + (double)rho;
+      out[side][1] = q[2][1] + (double)m1;
+      out[side][2] = q[2][2] + (double)m2;
+      out[side][3] = q[2][3] + (double)m3;
+      out[side][4] = q[2][4] + (double)E;
+    }
+  }
+  double rL, uL, vL, wL, pL, rR, uR, vR, wR, pR;
+  q_to_prim(g, qL, rL, uL, vL, wL, pL);
+  q_to_prim(g, qR, rR, uR, vR, wR, pR);
+  lf_flux(g, rR, uR, vR, wR, pR, rL, uL, vL, wL, pL, nx, ny, nz, f);
+}
+
+// face flux of the chosen arithmetic form: F = double (plain), float (precision)
+template <class F>
+__device__ __forceinline__ void face_flux_form(const GasT<double>& g, const double q[6][5],
+                                               const double pl[5], const double pr[5], double nx,
+                                               double ny, double nz, double f[5]) {
+  if constexpr (sizeof(F) == sizeof(double)) {
+    face_flux<double>(g, q, pl, pr, nx, ny, nz, f);
+  } else {
+    const GasT<F> gt{(F)g.gamma, (F)g.gam1, (F)g.gam1_inv};
+    face_flux_mixed<F>(gt, g, q, pl, pr, nx, ny, nz, f);
+  }
+}
+
 }  // namespace cbns

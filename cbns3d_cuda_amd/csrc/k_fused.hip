@@ -42,6 +42,7 @@ __device__ __forceinline__ void load_q(const double* __restrict__ Q, long S, int
   for (int n = 0; n < 5; n++) q[n] = (T)Q[n * S + o];
 }
 
+template <class F>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE_X, WPE_X))) k_xfaces(Consts c, Box box, const double* __restrict__ Q,
                                                 const double* __restrict__ met,
                                                 double* __restrict__ fx) {
@@ -59,7 +60,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE_X,
   const GasT<double> g{c.gamma, c.gam1, c.gam1_inv};
   q_to_prim(g, q[2], pl[0], pl[1], pl[2], pl[3], pl[4]);
   q_to_prim(g, q[3], pr[0], pr[1], pr[2], pr[3], pr[4]);
-  face_flux<double>(g, q, pl, pr, nx, ny, nz, f);
+  face_flux_form<F>(g, q, pl, pr, nx, ny, nz, f);
 #pragma unroll
   for (int n = 0; n < 5; n++) fx[n * S + a] = f[n];
 }
@@ -77,7 +78,7 @@ struct StageArgs {
 };
 
 // AX = 1: march along eta, rhs = dE + dF.  AX = 2: march along zeta, finish.
-template <int AX>
+template <int AX, class F>
 __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves_per_eu(WPE_M, WPE_M))) k_march(Consts c, StageArgs sa,
                                                             const double* __restrict__ met) {
   const int ax_t = AX == 1 ? 2 : 1;  // transverse axis other than i
@@ -111,7 +112,7 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
     double pl[5], pr[5];
     q_to_prim(g, q[2], pl[0], pl[1], pl[2], pl[3], pl[4]);
     q_to_prim(g, q[3], pr[0], pr[1], pr[2], pr[3], pr[4]);
-    face_flux<double>(g, q, pl, pr, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), fprev);
+    face_flux_form<F>(g, q, pl, pr, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), fprev);
   }
   for (int m = m0; m < m1; m++) {
     // shift the window by one node
@@ -129,7 +130,7 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
       double pl[5], pr[5];
       q_to_prim(g, q[2], pl[0], pl[1], pl[2], pl[3], pl[4]);
       q_to_prim(g, q[3], pr[0], pr[1], pr[2], pr[3], pr[4]);
-      face_flux<double>(g, q, pl, pr, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), f);
+      face_flux_form<F>(g, q, pl, pr, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), f);
     }
     if (AX == 1) {
 #pragma unroll
@@ -172,7 +173,8 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
 
 }  // namespace
 
-int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout) {
+template <class F>
+int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout) {
   const Consts& c = b->c;
   double* fx = b->flux;  // first 5 face arrays
   {
@@ -184,7 +186,7 @@ int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, d
     box.lo[0] = periodic(c, 0) ? -1 : 0;
     box.n[0] = c.IM - 1 + (periodic(c, 0) ? 2 : 0);  // faces -1..N-1 or 0..N-2
     ProfScope ps(b, CBNS_K_XFACES);
-    k_xfaces<<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fx);
+    k_xfaces<F><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fx);
     CBNS_HIP(b, hipGetLastError());
   }
   StageArgs sa{dt, stage, b->prim, b->Q, qin, qout, fx, b->rhs, nullptr};
@@ -195,11 +197,12 @@ int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, d
     dim3 grid((ni + MARCH_X - 1) / MARCH_X, (nt + MARCH_Y - 1) / MARCH_Y,
               (nm + CHUNK - 1) / CHUNK);
     ProfScope ps(b, CBNS_K_MARCH_ETA);
-    k_march<1><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
+    k_march<1, F><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
     CBNS_HIP(b, hipGetLastError());
   }
   if (c.viscous) {
-    int rc = launch_viscous(b, b->vtmp, nullptr, b->rhs, 1);
+    int rc = sizeof(F) == sizeof(double) ? launch_viscous(b, b->vtmp, nullptr, b->rhs, 1)
+                                         : launch_viscous_f32(b, b->vtmp, b->rhs);
     if (rc) return rc;
   }
   {
@@ -208,10 +211,15 @@ int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, d
     dim3 grid((ni + MARCH_X - 1) / MARCH_X, (nt + MARCH_Y - 1) / MARCH_Y,
               (nm + CHUNK - 1) / CHUNK);
     ProfScope ps(b, CBNS_K_MARCH_ZETA);
-    k_march<2><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
+    k_march<2, F><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
     CBNS_HIP(b, hipGetLastError());
   }
   return CBNS_OK;
+}
+
+int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout) {
+  return b->form == CBNS_FORM_FP32 ? fused_stage<float>(b, dt, stage, qin, qout)
+                                   : fused_stage<double>(b, dt, stage, qin, qout);
 }
 
 }  // namespace cbns

@@ -48,57 +48,61 @@ __device__ __forceinline__ double dgrad(const Consts& c, int a, int m, const dou
          1.0 / 12.0 * f[2 * st];
 }
 
+// T = double (plain form) or float (precision form: the index derivatives
+// are differenced in double and rounded, stresses, heat flux and fluxes are
+// formed and stored in float).
+template <class T>
 __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, int k,
                                                  const double* __restrict__ prim,
                                                  const double* __restrict__ met,
                                                  double* __restrict__ mu,
-                                                 double* __restrict__ vf) {
+                                                 T* __restrict__ vf) {
   const long S = c.S;
   const int o = idx(c, i, j, k);
   const double* U = prim + P_U * S + o;
   const double* V = prim + P_V * S + o;
   const double* W = prim + P_W * S + o;
-  const double* T = prim + P_T * S + o;
+  const double* TT = prim + P_T * S + o;
   const int sx = 1, sy = c.PX, sz = c.PXY;
-  double u_xi_l = dgrad(c, 0, i, U, sx), u_eta_l = dgrad(c, 1, j, U, sy), u_zeta_l = dgrad(c, 2, k, U, sz);
-  const double v_xi_l = dgrad(c, 0, i, V, sx), v_eta_l = dgrad(c, 1, j, V, sy), v_zeta_l = dgrad(c, 2, k, V, sz);
-  const double w_xi_l = dgrad(c, 0, i, W, sx), w_eta_l = dgrad(c, 1, j, W, sy), w_zeta_l = dgrad(c, 2, k, W, sz);
-  const double T_xi = dgrad(c, 0, i, T, sx), T_eta = dgrad(c, 1, j, T, sy), T_zeta = dgrad(c, 2, k, T, sz);
+  T u_xi_l = (T)dgrad(c, 0, i, U, sx), u_eta_l = (T)dgrad(c, 1, j, U, sy), u_zeta_l = (T)dgrad(c, 2, k, U, sz);
+  const T v_xi_l = (T)dgrad(c, 0, i, V, sx), v_eta_l = (T)dgrad(c, 1, j, V, sy), v_zeta_l = (T)dgrad(c, 2, k, V, sz);
+  const T w_xi_l = (T)dgrad(c, 0, i, W, sx), w_eta_l = (T)dgrad(c, 1, j, W, sy), w_zeta_l = (T)dgrad(c, 2, k, W, sz);
+  const T T_xi = (T)dgrad(c, 0, i, TT, sx), T_eta = (T)dgrad(c, 1, j, TT, sy), T_zeta = (T)dgrad(c, 2, k, TT, sz);
 
   // src/viscous_terms_kernel.cu:59-105
-  const double T_l = T[0];
+  const double T_l = TT[0];
   const double C_T_inf = c.C_T_inf;
-  double mu_l = (1.0 + C_T_inf) / (T_l + C_T_inf) * sqrt(T_l * T_l * T_l);
-  if (mu) mu[o] = mu_l;
-  const double xi_x_l = met[XI_X * S + o], xi_y_l = met[XI_Y * S + o], xi_z_l = met[XI_Z * S + o];
-  const double eta_x_l = met[ETA_X * S + o], eta_y_l = met[ETA_Y * S + o],
-               eta_z_l = met[ETA_Z * S + o];
-  const double zeta_x_l = met[ZETA_X * S + o], zeta_y_l = met[ZETA_Y * S + o],
-               zeta_z_l = met[ZETA_Z * S + o];
-  mu_l *= c.Re_inv;
-  const double txx = (2.0/3.0)*mu_l*(2*eta_x_l*u_eta_l - eta_y_l*v_eta_l - eta_z_l*w_eta_l + 2*u_xi_l*xi_x_l + 2*u_zeta_l*zeta_x_l - v_xi_l*xi_y_l - v_zeta_l*zeta_y_l - w_xi_l*xi_z_l - w_zeta_l*zeta_z_l);
-  const double txy = mu_l*(eta_x_l*v_eta_l + eta_y_l*u_eta_l + u_xi_l*xi_y_l + u_zeta_l*zeta_y_l + v_xi_l*xi_x_l + v_zeta_l*zeta_x_l);
-  const double tyy = (2.0/3.0)*mu_l*(-eta_x_l*u_eta_l + 2*eta_y_l*v_eta_l - eta_z_l*w_eta_l - u_xi_l*xi_x_l - u_zeta_l*zeta_x_l + 2*v_xi_l*xi_y_l + 2*v_zeta_l*zeta_y_l - w_xi_l*xi_z_l - w_zeta_l*zeta_z_l);
-  const double txz = mu_l*(eta_x_l*w_eta_l + eta_z_l*u_eta_l + u_xi_l*xi_z_l + u_zeta_l*zeta_z_l + w_xi_l*xi_x_l + w_zeta_l*zeta_x_l);
-  const double tyz = mu_l*(eta_y_l*w_eta_l + eta_z_l*v_eta_l + v_xi_l*xi_z_l + v_zeta_l*zeta_z_l + w_xi_l*xi_y_l + w_zeta_l*zeta_y_l);
-  const double tzz = (2.0/3.0)*mu_l*(-eta_x_l*u_eta_l - eta_y_l*v_eta_l + 2*eta_z_l*w_eta_l - u_xi_l*xi_x_l - u_zeta_l*zeta_x_l - v_xi_l*xi_y_l - v_zeta_l*zeta_y_l + 2*w_xi_l*xi_z_l + 2*w_zeta_l*zeta_z_l);
-  mu_l *= c.qcoef;
-  const double qx = mu_l * (xi_x_l * T_xi + eta_x_l * T_eta + zeta_x_l * T_zeta);
-  const double qy = mu_l * (xi_y_l * T_xi + eta_y_l * T_eta + zeta_y_l * T_zeta);
-  const double qz = mu_l * (xi_z_l * T_xi + eta_z_l * T_eta + zeta_z_l * T_zeta);
+  const double mu_d = (1.0 + C_T_inf) / (T_l + C_T_inf) * sqrt(T_l * T_l * T_l);
+  if (mu) mu[o] = mu_d;
+  const T xi_x_l = (T)met[XI_X * S + o], xi_y_l = (T)met[XI_Y * S + o], xi_z_l = (T)met[XI_Z * S + o];
+  const T eta_x_l = (T)met[ETA_X * S + o], eta_y_l = (T)met[ETA_Y * S + o],
+          eta_z_l = (T)met[ETA_Z * S + o];
+  const T zeta_x_l = (T)met[ZETA_X * S + o], zeta_y_l = (T)met[ZETA_Y * S + o],
+          zeta_z_l = (T)met[ZETA_Z * S + o];
+  T mu_l = (T)(mu_d * c.Re_inv);
+  const T txx = T(2.0/3.0)*mu_l*(2*eta_x_l*u_eta_l - eta_y_l*v_eta_l - eta_z_l*w_eta_l + 2*u_xi_l*xi_x_l + 2*u_zeta_l*zeta_x_l - v_xi_l*xi_y_l - v_zeta_l*zeta_y_l - w_xi_l*xi_z_l - w_zeta_l*zeta_z_l);
+  const T txy = mu_l*(eta_x_l*v_eta_l + eta_y_l*u_eta_l + u_xi_l*xi_y_l + u_zeta_l*zeta_y_l + v_xi_l*xi_x_l + v_zeta_l*zeta_x_l);
+  const T tyy = T(2.0/3.0)*mu_l*(-eta_x_l*u_eta_l + 2*eta_y_l*v_eta_l - eta_z_l*w_eta_l - u_xi_l*xi_x_l - u_zeta_l*zeta_x_l + 2*v_xi_l*xi_y_l + 2*v_zeta_l*zeta_y_l - w_xi_l*xi_z_l - w_zeta_l*zeta_z_l);
+  const T txz = mu_l*(eta_x_l*w_eta_l + eta_z_l*u_eta_l + u_xi_l*xi_z_l + u_zeta_l*zeta_z_l + w_xi_l*xi_x_l + w_zeta_l*zeta_x_l);
+  const T tyz = mu_l*(eta_y_l*w_eta_l + eta_z_l*v_eta_l + v_xi_l*xi_z_l + v_zeta_l*zeta_z_l + w_xi_l*xi_y_l + w_zeta_l*zeta_y_l);
+  const T tzz = T(2.0/3.0)*mu_l*(-eta_x_l*u_eta_l - eta_y_l*v_eta_l + 2*eta_z_l*w_eta_l - u_xi_l*xi_x_l - u_zeta_l*zeta_x_l - v_xi_l*xi_y_l - v_zeta_l*zeta_y_l + 2*w_xi_l*xi_z_l + 2*w_zeta_l*zeta_z_l);
+  mu_l *= (T)c.qcoef;
+  const T qx = mu_l * (xi_x_l * T_xi + eta_x_l * T_eta + zeta_x_l * T_zeta);
+  const T qy = mu_l * (xi_y_l * T_xi + eta_y_l * T_eta + zeta_y_l * T_zeta);
+  const T qz = mu_l * (xi_z_l * T_xi + eta_z_l * T_eta + zeta_z_l * T_zeta);
 
   // src/viscous_flux_kernel.cu:57-110
   const double Jac_a = met[JAC * S + o];
-  const double uu = U[0], vv = V[0], ww = W[0];
-  const double ex = uu * txx + vv * txy + ww * txz - qx;
-  const double ey = uu * txy + vv * tyy + ww * tyz - qy;
-  const double ez = uu * txz + vv * tyz + ww * tzz - qz;
+  const T uu = (T)U[0], vv = (T)V[0], ww = (T)W[0];
+  const T ex = uu * txx + vv * txy + ww * txz - qx;
+  const T ey = uu * txy + vv * tyy + ww * tyz - qy;
+  const T ez = uu * txz + vv * tyz + ww * tzz - qz;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const double n_x = met[(3 * d + 0) * S + o] * Jac_a;
-    const double n_y = met[(3 * d + 1) * S + o] * Jac_a;
-    const double n_z = met[(3 * d + 2) * S + o] * Jac_a;
-    double* E = vf + (long)(4 * d) * S + o;
+    const T n_x = (T)(met[(3 * d + 0) * S + o] * Jac_a);
+    const T n_y = (T)(met[(3 * d + 1) * S + o] * Jac_a);
+    const T n_z = (T)(met[(3 * d + 2) * S + o] * Jac_a);
+    T* E = vf + (long)(4 * d) * S + o;
     E[0] = n_x * txx + n_y * txy + n_z * txz;
     E[S] = n_x * txy + n_y * tyy + n_z * tyz;
     E[2 * S] = n_x * txz + n_y * tyz + n_z * tzz;
@@ -106,54 +110,58 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
   }
 }
 
+template <class T>
 __global__ void __launch_bounds__(256) k_visc_fluxes(Consts c, TileMarch tm,
                                                      const double* __restrict__ prim,
                                                      const double* __restrict__ met,
                                                      double* __restrict__ mu,
-                                                     double* __restrict__ vf) {
+                                                     T* __restrict__ vf) {
   int i, j, k0, k1;
   if (!tile_march_node(tm, i, j, k0, k1)) return;
-  for (int k = k0; k < k1; k++) visc_fluxes_node(c, i, j, k, prim, met, mu, vf);
+  for (int k = k0; k < k1; k++) visc_fluxes_node<T>(c, i, j, k, prim, met, mu, vf);
 }
 
 // src/viscous_contribution_kernel.cu:35-67.  mode 0: dfv[e] = div; mode 1:
 // out[(e+1)] -= div (the fused stage's right-hand side, momentum and energy).
+template <class T>
 __device__ __forceinline__ void visc_div_node(const Consts& c, int i, int j, int k,
-                                              const double* __restrict__ vf,
+                                              const T* __restrict__ vf,
                                               double* __restrict__ out, int mode) {
   const long S = c.S;
   const int sh = c.visc_centered ? 0 : -1;
-  const double c0 = 1.0 / 12.0, c1 = -2.0 / 3.0;
+  const T c0 = T(1.0 / 12.0), c1 = T(-2.0 / 3.0);
   const int ox = idx(c, i, j + sh, k + sh), oy = idx(c, i + sh, j, k + sh),
             oz = idx(c, i + sh, j + sh, k);
   const int sx = 1, sy = c.PX, sz = c.PXY;
   const int o = idx(c, i, j, k);
 #pragma unroll
   for (int e = 0; e < 4; e++) {
-    const double* Ev = vf + (long)e * S;
-    const double* Fv = vf + (long)(4 + e) * S;
-    const double* Gv = vf + (long)(8 + e) * S;
-    double df = c0 * Ev[ox - 2 * sx] + c1 * Ev[ox - sx] - c1 * Ev[ox + sx] - c0 * Ev[ox + 2 * sx];
+    const T* Ev = vf + (long)e * S;
+    const T* Fv = vf + (long)(4 + e) * S;
+    const T* Gv = vf + (long)(8 + e) * S;
+    T df = c0 * Ev[ox - 2 * sx] + c1 * Ev[ox - sx] - c1 * Ev[ox + sx] - c0 * Ev[ox + 2 * sx];
     df += c0 * Fv[oy - 2 * sy] + c1 * Fv[oy - sy] - c1 * Fv[oy + sy] - c0 * Fv[oy + 2 * sy];
     df += c0 * Gv[oz - 2 * sz] + c1 * Gv[oz - sz] - c1 * Gv[oz + sz] - c0 * Gv[oz + 2 * sz];
-    if (mode == 0) out[e * S + o] = df;
-    else out[(e + 1) * S + o] -= df;
+    if (mode == 0) out[e * S + o] = (double)df;
+    else out[(e + 1) * S + o] -= (double)df;
   }
 }
 
+template <class T>
 __global__ void __launch_bounds__(256) k_visc_div(Consts c, TileMarch tm,
-                                                  const double* __restrict__ vf,
+                                                  const T* __restrict__ vf,
                                                   double* __restrict__ out, int mode) {
   int i, j, k0, k1;
   if (!tile_march_node(tm, i, j, k0, k1)) return;
-  for (int k = k0; k < k1; k++) visc_div_node(c, i, j, k, vf, out, mode);
+  for (int k = k0; k < k1; k++) visc_div_node<T>(c, i, j, k, vf, out, mode);
 }
 
 }  // namespace
 
-// vf: 12*S scratch; mu may be null.  mode 0 writes dfv (4*S) to `out`,
-// mode 1 subtracts the divergence from the right-hand side `out` (5*S).
-int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode) {
+// vf: 12*S scratch (of T); mu may be null.  mode 0 writes dfv (4*S) to
+// `out`, mode 1 subtracts the divergence from the right-hand side (5*S).
+template <class T>
+int viscous(cbns_block* b, T* vf, double* mu, double* out, int mode) {
   const Consts& c = b->c;
   Box vb;
   for (int a = 0; a < 3; a++) {
@@ -163,8 +171,8 @@ int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode)
   {
     ProfScope ps(b, CBNS_K_VISC_FLUX);
     const TileMarch tm = tile_march(vb, VISC_CHUNK);
-    k_visc_fluxes<<<tile_march_blocks(tm), box_block(), 0, b->stream>>>(c, tm, b->prim, b->met,
-                                                                       mu, vf);
+    k_visc_fluxes<T><<<tile_march_blocks(tm), box_block(), 0, b->stream>>>(c, tm, b->prim,
+                                                                          b->met, mu, vf);
     CBNS_HIP(b, hipGetLastError());
   }
   Box ub;
@@ -174,9 +182,18 @@ int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode)
   }
   ProfScope ps(b, CBNS_K_VISC_DIV);
   const TileMarch tm = tile_march(ub, VISC_CHUNK);
-  k_visc_div<<<tile_march_blocks(tm), box_block(), 0, b->stream>>>(c, tm, vf, out, mode);
+  k_visc_div<T><<<tile_march_blocks(tm), box_block(), 0, b->stream>>>(c, tm, vf, out, mode);
   CBNS_HIP(b, hipGetLastError());
   return CBNS_OK;
+}
+
+int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode) {
+  return viscous<double>(b, vf, mu, out, mode);
+}
+
+// precision form: the same scratch holds 12*S floats
+int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs) {
+  return viscous<float>(b, reinterpret_cast<float*>(scratch), nullptr, rhs, 1);
 }
 
 }  // namespace cbns
