@@ -20,7 +20,8 @@ G = 4
 NG_REF = 2
 BC_REF, BC_PERIODIC = 0, 1
 BUF_Q, BUF_QP = 0, 1
-FORM_FP64, FORM_FP32 = 0, 1
+FORM_FP64, FORM_FP32, FORM_HYBRID = 0, 1, 2
+MLP_NFEAT, MLP_NPARAM = 32, 2145
 F_Q, F_QP, F_PRIM, F_MET, F_MU, F_FLUX, F_DFV = 0, 5, 10, 16, 26, 27, 42
 ERR_NAMES = {1: "ARG", 2: "HIP", 3: "NAN", 4: "MESH", 5: "STATE", 6: "NO_DEVICE"}
 
@@ -70,8 +71,13 @@ EXPORTS = [
     "cbns_calc_viscous_flux_contribution", "cbns_update_rk3", "cbns_rk3_stage",
     "cbns_rk3_step", "cbns_set_form", "cbns_get_field", "cbns_set_field",
     "cbns_synchronize", "cbns_field_size", "cbns_set_profiling", "cbns_kernel_time",
+    "cbns_face_count", "cbns_face_samples", "cbns_set_indicator", "cbns_flag_count",
+    "cbns_get_flags", "cbns_dev_alloc", "cbns_dev_free", "cbns_dev_upload", "cbns_dev_download",
+    "cbns_gather_rows",
+    "cbns_mlp_create", "cbns_mlp_destroy", "cbns_mlp_last_error", "cbns_mlp_set_params",
+    "cbns_mlp_get_params", "cbns_mlp_forward", "cbns_mlp_loss_grad", "cbns_mlp_adam_step",
 ]
-KERNELS = ["xfaces", "march_eta", "march_zeta", "visc_flux", "visc_div", "bc", "dt"]
+KERNELS = ["xfaces", "march_eta", "march_zeta", "visc_flux", "visc_div", "bc", "dt", "indicator"]
 
 
 def lib():
@@ -85,6 +91,7 @@ def lib():
     D = C.POINTER(C.c_double)
     B = C.c_void_p
     I = C.c_int32
+    FP = C.POINTER(C.c_float)
     sig = {
         "cbns_create": ([C.POINTER(Params), C.POINTER(C.c_void_p)], C.c_int),
         "cbns_destroy": ([B], None),
@@ -113,6 +120,24 @@ def lib():
         "cbns_field_size": ([B], C.c_int64),
         "cbns_set_profiling": ([B, I], C.c_int),
         "cbns_kernel_time": ([B, I, I, D, C.POINTER(C.c_int64)], C.c_int),
+        "cbns_face_count": ([B, I, C.POINTER(C.c_int64)], C.c_int),
+        "cbns_face_samples": ([B, I, I, C.c_void_p, C.c_void_p], C.c_int),
+        "cbns_set_indicator": ([B, FP, C.c_float, I], C.c_int),
+        "cbns_flag_count": ([B, C.POINTER(C.c_int64)], C.c_int),
+        "cbns_get_flags": ([B, C.c_void_p], C.c_int),
+        "cbns_dev_alloc": ([C.c_int64, C.POINTER(C.c_void_p)], C.c_int),
+        "cbns_dev_free": ([C.c_void_p], C.c_int),
+        "cbns_dev_upload": ([C.c_void_p, C.c_void_p, C.c_int64], C.c_int),
+        "cbns_dev_download": ([C.c_void_p, C.c_void_p, C.c_int64], C.c_int),
+        "cbns_gather_rows": ([C.c_void_p, C.c_void_p, C.c_int64, I, C.c_void_p], C.c_int),
+        "cbns_mlp_create": ([C.POINTER(C.c_void_p)], C.c_int),
+        "cbns_mlp_destroy": ([B], None),
+        "cbns_mlp_last_error": ([B], C.c_char_p),
+        "cbns_mlp_set_params": ([B, FP], C.c_int),
+        "cbns_mlp_get_params": ([B, FP], C.c_int),
+        "cbns_mlp_forward": ([B, C.c_void_p, C.c_int64, C.c_void_p], C.c_int),
+        "cbns_mlp_loss_grad": ([B, C.c_void_p, C.c_void_p, C.c_int64, C.c_float, D, FP], C.c_int),
+        "cbns_mlp_adam_step": ([B, C.c_float, C.c_float, C.c_float, C.c_float], C.c_int),
     }
     for name, (args, res) in sig.items():
         fn = getattr(L, name)
@@ -125,6 +150,11 @@ def lib():
 def _ptr(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fptr(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 class Block:
@@ -256,3 +286,144 @@ class Block:
         arr = arr.reshape((n,) + self.shape)
         for q in range(n):
             self._chk(lib().cbns_set_field(self._h, f + q, _ptr(arr[q])))
+
+    # -- learned troubled-face indicator -----------------------------------
+    # d_* arguments are device addresses (ints) of this library's HIP
+    # runtime: DeviceArray.ptr / .at(offset).  Pointers from another HIP
+    # runtime in the same process (a framework's bundled one) must not be
+    # passed: the two runtimes do not share allocations.
+    def face_count(self, axis):
+        n = C.c_int64(0)
+        self._chk(lib().cbns_face_count(self._h, axis, C.byref(n)))
+        return n.value
+
+    def face_samples(self, axis, d_feat, d_label, buf=BUF_Q):
+        """Features [n][32] and WENO-weight labels [n] of axis `axis`."""
+        self._chk(lib().cbns_face_samples(self._h, axis, buf, d_feat, d_label))
+
+    def set_indicator(self, params, threshold=0.0, per_stage=False, count=False):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.size == MLP_NPARAM
+        opts = (1 if per_stage else 0) | (2 if count else 0)
+        self._chk(lib().cbns_set_indicator(self._h, _fptr(params), float(threshold), opts))
+
+    def get_flags(self):
+        """troubled-face bytes, shape (3,) + ghosted shape"""
+        out = np.zeros((3,) + self.shape, dtype=np.uint8)
+        self._chk(lib().cbns_get_flags(self._h, out.ctypes.data))
+        return out
+
+    def flag_count(self):
+        out = (C.c_int64 * 3)()
+        self._chk(lib().cbns_flag_count(self._h, out))
+        return list(out)
+
+
+class DeviceArray:
+    """A device buffer of libcbns's own HIP runtime (shape + numpy dtype).
+    Its address (`ptr`) is what the d_* arguments of the C ABI take."""
+
+    def __init__(self, shape, dtype=np.float32):
+        self.shape = tuple(np.atleast_1d(shape))
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = C.c_void_p()
+        rc = lib().cbns_dev_alloc(max(self.nbytes, 1), C.byref(p))
+        if rc:
+            raise CbnsError(rc, "cbns_dev_alloc(%d) failed" % self.nbytes)
+        self.ptr = p.value
+
+    @classmethod
+    def from_host(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.shape, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.nbytes <= self.nbytes
+        rc = lib().cbns_dev_upload(self.ptr, a.ctypes.data, a.nbytes)
+        if rc:
+            raise CbnsError(rc, "cbns_dev_upload failed")
+
+    def download(self, count=None):
+        n = int(np.prod(self.shape)) if count is None else int(count)
+        out = np.empty(n, dtype=self.dtype)
+        rc = lib().cbns_dev_download(out.ctypes.data, self.ptr, out.nbytes)
+        if rc:
+            raise CbnsError(rc, "cbns_dev_download failed")
+        return out.reshape(self.shape) if count is None else out
+
+    def at(self, offset):
+        """address of element `offset` (flat)"""
+        return self.ptr + int(offset) * self.dtype.itemsize
+
+    def free(self):
+        if self.ptr:
+            lib().cbns_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def gather_rows(d_src, d_idx, n, row, d_dst):
+    rc = lib().cbns_gather_rows(d_src, d_idx, int(n), int(row), d_dst)
+    if rc:
+        raise CbnsError(rc, "cbns_gather_rows failed")
+
+
+class Mlp:
+    """The indicator network alone, trained on the GPU (cbns_mlp_*)."""
+
+    def __init__(self, params=None):
+        h = C.c_void_p()
+        rc = lib().cbns_mlp_create(C.byref(h))
+        if rc:
+            raise CbnsError(rc, "cbns_mlp_create failed")
+        self._h = h
+        if params is not None:
+            self.set_params(params)
+
+    def _chk(self, rc):
+        if rc:
+            msg = lib().cbns_mlp_last_error(self._h)
+            raise CbnsError(rc, msg.decode() if msg else "")
+
+    def free(self):
+        if self._h:
+            lib().cbns_mlp_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+    def set_params(self, params):
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        assert params.size == MLP_NPARAM
+        self._chk(lib().cbns_mlp_set_params(self._h, _fptr(params)))
+
+    def get_params(self):
+        out = np.zeros(MLP_NPARAM, dtype=np.float32)
+        self._chk(lib().cbns_mlp_get_params(self._h, _fptr(out)))
+        return out
+
+    def forward(self, d_feat, n, d_logit):
+        self._chk(lib().cbns_mlp_forward(self._h, d_feat, n, d_logit))
+
+    def loss_grad(self, d_feat, d_target, n, w_pos=1.0, want_grad=False):
+        loss = C.c_double(0.0)
+        g = np.zeros(MLP_NPARAM, dtype=np.float32) if want_grad else None
+        self._chk(lib().cbns_mlp_loss_grad(self._h, d_feat, d_target, n, float(w_pos),
+                                           C.byref(loss), None if g is None else _fptr(g)))
+        return (loss.value, g) if want_grad else loss.value
+
+    def adam_step(self, lr, beta1=0.9, beta2=0.999, eps=1e-8):
+        self._chk(lib().cbns_mlp_adam_step(self._h, lr, beta1, beta2, eps))

@@ -198,6 +198,8 @@ void cbns_destroy(cbns_block* b) {
                   b->red};
   for (double* p : ps)
     if (p) (void)hipFree(p);
+  for (void* p : {(void*)b->flags, (void*)b->ind_P, b->ind_pack, (void*)b->ind_count})
+    if (p) (void)hipFree(p);
   if (b->h_scalar) (void)hipHostFree(b->h_scalar);
   for (auto& v : b->prof.live)
     for (auto& ev : v) b->prof.pool.push_back(ev);
@@ -434,8 +436,71 @@ int cbns_rk3_step(cbns_block* b, double t_end, double* t_cur, double* dt_out) {
 
 int cbns_set_form(cbns_block* b, int32_t form) {
   if (!b) return CBNS_ERR_ARG;
-  if (form != CBNS_FORM_FP64 && form != CBNS_FORM_FP32) return fail(b, CBNS_ERR_ARG, "bad form");
+  if (form < 0 || form > (CBNS_FORM_FP32 | CBNS_FORM_HYBRID)) return fail(b, CBNS_ERR_ARG, "bad form");
+  if ((form & CBNS_FORM_HYBRID) && !b->have_indicator)
+    return fail(b, CBNS_ERR_STATE, "hybrid form needs cbns_set_indicator first");
   b->form = form;
+  b->flags_valid = false;
+  return CBNS_OK;
+}
+
+int cbns_face_count(cbns_block* b, int32_t axis, int64_t* n) {
+  if (!b || !n || axis < 0 || axis > 2) return CBNS_ERR_ARG;
+  *n = face_count(b, axis);
+  return CBNS_OK;
+}
+
+int cbns_face_samples(cbns_block* b, int32_t axis, int32_t buf, float* d_feat, float* d_label) {
+  int rc = check_block(b);
+  if (rc) return rc;
+  if (axis < 0 || axis > 2 || (buf != CBNS_BUF_Q && buf != CBNS_BUF_QP) || !d_feat || !d_label)
+    return fail(b, CBNS_ERR_ARG, "bad face_samples argument");
+  if ((rc = launch_face_samples(b, axis, buf_ptr(b, buf), d_feat, d_label))) return rc;
+  CBNS_HIP(b, hipStreamSynchronize(b->stream));
+  return CBNS_OK;
+}
+
+int cbns_set_indicator(cbns_block* b, const float* params, float threshold, int32_t options) {
+  if (!b || !params || !std::isfinite(threshold) || options < 0 || options > 3) return CBNS_ERR_ARG;
+  CBNS_HIP(b, hipSetDevice(b->device));
+  if (!b->flags) {
+    CBNS_HIP(b, hipMalloc((void**)&b->flags, 3 * (size_t)b->c.S));
+    CBNS_HIP(b, hipMemsetAsync(b->flags, 0, 3 * (size_t)b->c.S, b->stream));
+  }
+  if (!b->ind_P) CBNS_HIP(b, hipMalloc((void**)&b->ind_P, 2176 * sizeof(float)));
+  if (!b->ind_pack) CBNS_HIP(b, hipMalloc(&b->ind_pack, 768 * 8));
+  if (!b->ind_count) {
+    CBNS_HIP(b, hipMalloc((void**)&b->ind_count, 4 * sizeof(unsigned)));
+    CBNS_HIP(b, hipMemsetAsync(b->ind_count, 0, 4 * sizeof(unsigned), b->stream));
+  }
+  std::vector<float> p(params, params + CBNS_MLP_NPARAM);
+  for (int r = 0; r < 32; r++) p[r * 32 + 30] = p[r * 32 + 31] = 0.0f;  // padding inputs
+  int rc = upload_indicator(b, p.data());
+  if (rc) return rc;
+  b->ind_thresh = threshold;
+  b->ind_per_stage = (options & 1) != 0;
+  b->ind_count_on = (options & 2) != 0;
+  b->have_indicator = true;
+  b->flags_valid = false;
+  return CBNS_OK;
+}
+
+int cbns_get_flags(cbns_block* b, uint8_t* host) {
+  if (!b || !host) return CBNS_ERR_ARG;
+  if (!b->flags) return fail(b, CBNS_ERR_STATE, "no indicator loaded");
+  CBNS_HIP(b, hipSetDevice(b->device));
+  CBNS_HIP(b, hipMemcpyAsync(host, b->flags, 3 * (size_t)b->c.S, hipMemcpyDeviceToHost, b->stream));
+  CBNS_HIP(b, hipStreamSynchronize(b->stream));
+  return CBNS_OK;
+}
+
+int cbns_flag_count(cbns_block* b, int64_t counts[3]) {
+  if (!b || !counts) return CBNS_ERR_ARG;
+  if (!b->ind_count) return fail(b, CBNS_ERR_STATE, "no indicator loaded");
+  unsigned h[4] = {0, 0, 0, 0};
+  CBNS_HIP(b, hipMemcpyAsync(h, b->ind_count, 4 * sizeof(unsigned), hipMemcpyDeviceToHost, b->stream));
+  CBNS_HIP(b, hipStreamSynchronize(b->stream));
+  for (int a = 0; a < 3; a++) counts[a] = h[a];
   return CBNS_OK;
 }
 

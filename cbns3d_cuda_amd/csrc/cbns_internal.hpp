@@ -168,6 +168,16 @@ struct cbns_block {
   double* red = nullptr;     // reduction scratch
   int red_n = 0;
   double* h_scalar = nullptr;  // pinned host scalars [4]
+  // learned hybrid (cbns_set_indicator)
+  bool have_indicator = false;
+  bool flags_valid = false;
+  bool ind_count_on = false;
+  bool ind_per_stage = false;
+  float ind_thresh = 0.0f;
+  float* ind_P = nullptr;        // network parameters (fp32)
+  void* ind_pack = nullptr;      // bf16 operand tiles (3 x 256 x 8 bytes)
+  unsigned* ind_count = nullptr; // troubled faces per axis [3]
+  uint8_t* flags = nullptr;      // 3*S troubled-face bytes, face m at node m
   std::string err;
 };
 
@@ -212,5 +222,9 @@ int launch_update_rk3(cbns_block* b, double dt, int stage);
 int launch_calc_dt(cbns_block* b, double* dt_host);
 int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout);
 int launch_fill(hipStream_t s, double* p, long n, double v);
+long face_count(const cbns_block* b, int axis);
+int launch_face_samples(cbns_block* b, int axis, const double* Q, float* feat, float* label);
+int launch_indicator(cbns_block* b, const double* Q);
+int upload_indicator(cbns_block* b, const float* params);
 
 }  // namespace cbns

@@ -18,6 +18,7 @@
 // boundary rules write Q first and derive the primitives from it).
 #include "cbns_internal.hpp"
 #include "face_math.hpp"
+#include "learned.hpp"
 
 namespace cbns {
 
@@ -42,9 +43,27 @@ __device__ __forceinline__ void load_q(const double* __restrict__ Q, long S, int
   for (int n = 0; n < 5; n++) q[n] = (T)Q[n * S + o];
 }
 
-template <class F>
+// Face flux of the stage's form.  HYB: faces whose troubled flag (face m-1,
+// m or m+1 along the axis: one-cell dilation) is set run characteristic
+// WENO5, the others the linear-weight reconstruction.
+template <class F, bool HYB>
+__device__ __forceinline__ void stage_face(const GasT<double>& g, const double q[6][5],
+                                           double nx, double ny, double nz, bool troubled,
+                                           double f[5]) {
+  if (!HYB || troubled) {
+    double pl[5], pr[5];
+    q_to_prim(g, q[2], pl[0], pl[1], pl[2], pl[3], pl[4]);
+    q_to_prim(g, q[3], pr[0], pr[1], pr[2], pr[3], pr[4]);
+    face_flux_form<F>(g, q, pl, pr, nx, ny, nz, f);
+  } else {
+    face_linear<F>(g, q, nx, ny, nz, f);
+  }
+}
+
+template <class F, bool HYB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE_X, WPE_X))) k_xfaces(Consts c, Box box, const double* __restrict__ Q,
                                                 const double* __restrict__ met,
+                                                const uint8_t* __restrict__ flags,
                                                 double* __restrict__ fx) {
   int i, j, k;
   if (!box_node3(box, i, j, k)) return;
@@ -54,13 +73,12 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE_X,
   const double nx = 0.5 * (met[XI_X * S + a] * Ja + met[XI_X * S + b] * Jb);
   const double ny = 0.5 * (met[XI_Y * S + a] * Ja + met[XI_Y * S + b] * Jb);
   const double nz = 0.5 * (met[XI_Z * S + a] * Ja + met[XI_Z * S + b] * Jb);
-  double q[6][5], pl[5], pr[5], f[5];
+  double q[6][5], f[5];
 #pragma unroll
   for (int s = 0; s < 6; s++) load_q(Q, S, a + (s - 2), q[s]);
   const GasT<double> g{c.gamma, c.gam1, c.gam1_inv};
-  q_to_prim(g, q[2], pl[0], pl[1], pl[2], pl[3], pl[4]);
-  q_to_prim(g, q[3], pr[0], pr[1], pr[2], pr[3], pr[4]);
-  face_flux_form<F>(g, q, pl, pr, nx, ny, nz, f);
+  const bool troubled = HYB && (flags[a - 1] | flags[a] | flags[a + 1]);
+  stage_face<F, HYB>(g, q, nx, ny, nz, troubled, f);
 #pragma unroll
   for (int n = 0; n < 5; n++) fx[n * S + a] = f[n];
 }
@@ -75,10 +93,11 @@ struct StageArgs {
   const double* fx;     // xi-face fluxes (AX == 1)
   double* rhs;          // dE + dF (+ viscous)
   const double* dfv;    // viscous divergence or nullptr
+  const uint8_t* flags; // troubled faces of the march axis (HYB)
 };
 
 // AX = 1: march along eta, rhs = dE + dF.  AX = 2: march along zeta, finish.
-template <int AX, class F>
+template <int AX, class F, bool HYB>
 __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves_per_eu(WPE_M, WPE_M))) k_march(Consts c, StageArgs sa,
                                                             const double* __restrict__ met) {
   const int ax_t = AX == 1 ? 2 : 1;  // transverse axis other than i
@@ -108,12 +127,16 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
   double ax = nX[o] * ja, ay = nY[o] * ja, az = nZ[o] * ja;
   double bx = nX[o + st] * jb, by = nY[o + st] * jb, bz = nZ[o + st] * jb;
   double fprev[5];
-  {
-    double pl[5], pr[5];
-    q_to_prim(g, q[2], pl[0], pl[1], pl[2], pl[3], pl[4]);
-    q_to_prim(g, q[3], pr[0], pr[1], pr[2], pr[3], pr[4]);
-    face_flux_form<F>(g, q, pl, pr, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), fprev);
+  // troubled flags of faces m-1, m, m+1 (face m between nodes m and m+1)
+  const uint8_t* __restrict__ fl = sa.flags;
+  uint8_t t0 = 0, t1 = 0, t2 = 0;
+  if (HYB) {
+    t0 = fl[base + (m0 - 2) * st];
+    t1 = fl[base + (m0 - 1) * st];
+    t2 = fl[base + m0 * st];
   }
+  stage_face<F, HYB>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), t0 | t1 | t2,
+                     fprev);
   for (int m = m0; m < m1; m++) {
     // shift the window by one node
 #pragma unroll
@@ -126,12 +149,12 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
     jb = J[o + st];
     bx = nX[o + st] * jb; by = nY[o + st] * jb; bz = nZ[o + st] * jb;
     double f[5];
-    {
-      double pl[5], pr[5];
-      q_to_prim(g, q[2], pl[0], pl[1], pl[2], pl[3], pl[4]);
-      q_to_prim(g, q[3], pr[0], pr[1], pr[2], pr[3], pr[4]);
-      face_flux_form<F>(g, q, pl, pr, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), f);
+    if (HYB) {
+      t0 = t1;
+      t1 = t2;
+      t2 = fl[o + st];
     }
+    stage_face<F, HYB>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), t0 | t1 | t2, f);
     if (AX == 1) {
 #pragma unroll
       for (int n = 0; n < 5; n++) {
@@ -173,10 +196,16 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
 
 }  // namespace
 
-template <class F>
+template <class F, bool HYB>
 int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout) {
   const Consts& c = b->c;
   double* fx = b->flux;  // first 5 face arrays
+  if (HYB && (stage == 1 || b->ind_per_stage || !b->flags_valid)) {
+    int rc = launch_indicator(b, qin);
+    if (rc) return rc;
+  }
+  const long S = c.S;
+  const uint8_t* fl = HYB ? b->flags : nullptr;
   {
     Box box;
     for (int a = 0; a < 3; a++) {
@@ -186,10 +215,10 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
     box.lo[0] = periodic(c, 0) ? -1 : 0;
     box.n[0] = c.IM - 1 + (periodic(c, 0) ? 2 : 0);  // faces -1..N-1 or 0..N-2
     ProfScope ps(b, CBNS_K_XFACES);
-    k_xfaces<F><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fx);
+    k_xfaces<F, HYB><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fl, fx);
     CBNS_HIP(b, hipGetLastError());
   }
-  StageArgs sa{dt, stage, b->prim, b->Q, qin, qout, fx, b->rhs, nullptr};
+  StageArgs sa{dt, stage, b->prim, b->Q, qin, qout, fx, b->rhs, nullptr, HYB ? fl + S : nullptr};
   const int ni = upd_hi(c, 0) - upd_lo(c, 0) + 1;
   {
     const int nt = upd_hi(c, 2) - upd_lo(c, 2) + 1;
@@ -197,7 +226,7 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
     dim3 grid((ni + MARCH_X - 1) / MARCH_X, (nt + MARCH_Y - 1) / MARCH_Y,
               (nm + CHUNK - 1) / CHUNK);
     ProfScope ps(b, CBNS_K_MARCH_ETA);
-    k_march<1, F><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
+    k_march<1, F, HYB><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
     CBNS_HIP(b, hipGetLastError());
   }
   if (c.viscous) {
@@ -211,15 +240,20 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
     dim3 grid((ni + MARCH_X - 1) / MARCH_X, (nt + MARCH_Y - 1) / MARCH_Y,
               (nm + CHUNK - 1) / CHUNK);
     ProfScope ps(b, CBNS_K_MARCH_ZETA);
-    k_march<2, F><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
+    if (HYB) sa.flags = fl + 2 * S;
+    k_march<2, F, HYB><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
     CBNS_HIP(b, hipGetLastError());
   }
   return CBNS_OK;
 }
 
 int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout) {
-  return b->form == CBNS_FORM_FP32 ? fused_stage<float>(b, dt, stage, qin, qout)
-                                   : fused_stage<double>(b, dt, stage, qin, qout);
+  switch (b->form) {
+    case CBNS_FORM_FP32: return fused_stage<float, false>(b, dt, stage, qin, qout);
+    case CBNS_FORM_HYBRID: return fused_stage<double, true>(b, dt, stage, qin, qout);
+    case CBNS_FORM_FP32 | CBNS_FORM_HYBRID: return fused_stage<float, true>(b, dt, stage, qin, qout);
+    default: return fused_stage<double, false>(b, dt, stage, qin, qout);
+  }
 }
 
 }  // namespace cbns

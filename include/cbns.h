@@ -47,9 +47,11 @@ enum { CBNS_BC_REF = 0,        /* the reference's RAE2822 C-grid rule for the ax
 /* stage buffer selector: the reference's data.Q / data.Q_p */
 enum { CBNS_BUF_Q = 0, CBNS_BUF_QP = 1 };
 
-/* arithmetic form of the fused stage */
+/* arithmetic form of the fused stage: FP64 or FP32, optionally with the
+ * learned hybrid (CBNS_FORM_HYBRID bit; needs cbns_set_indicator first) */
 enum { CBNS_FORM_FP64 = 0,     /* plain FP64                                 */
-       CBNS_FORM_FP32 = 1 };   /* FP32 face/viscous arithmetic, FP64 state   */
+       CBNS_FORM_FP32 = 1,     /* FP32 face/viscous arithmetic, FP64 state   */
+       CBNS_FORM_HYBRID = 2 }; /* flagged faces WENO5, the rest linear       */
 
 /* field ids for cbns_get_field / cbns_set_field (ghosted SoA, CBNS_G) */
 enum {
@@ -157,10 +159,77 @@ enum {
   CBNS_K_VISC_DIV = 4,   /* viscous divergence into the stage rhs   */
   CBNS_K_BC = 5,         /* boundary / ghost fill                   */
   CBNS_K_DT = 6,         /* CFL time step reduction                 */
-  CBNS_K_COUNT = 7
+  CBNS_K_INDICATOR = 7,  /* learned troubled-face flags (hybrid)    */
+  CBNS_K_COUNT = 8
 };
 int cbns_set_profiling(cbns_block* blk, int32_t on);
 int cbns_kernel_time(cbns_block* blk, int32_t kernel, int32_t reset, double* ms, int64_t* launches);
+
+/* ---- learned troubled-face indicator (no counterpart in the reference) ----
+ * A 30(32)->32->32->1 ReLU network reads, per face, the 30 normalised
+ * characteristic stencil differences and predicts where the WENO5 nonlinear
+ * weights of src/weno.cu:13-44 leave the linear ones.  Parameters: one float
+ * array of CBNS_MLP_NPARAM: W1[32][32] (hidden x input; input columns 30, 31
+ * are padding and stay zero), b1[32], W2[32][32], b2[32], w3[32], b3.
+ * Arguments named d_* are DEVICE pointers on the block's / network's device;
+ * the caller makes their contents ready (synchronises its own stream) before
+ * the call; every call below returns after its device work is complete. */
+#define CBNS_MLP_NFEAT 32    /* floats per feature row (30 used)  */
+#define CBNS_MLP_NPARAM 2145 /* floats of the parameter array      */
+
+/* faces of one axis the flux kernels compute (the sample count of
+ * cbns_face_samples): (faces along axis) x (updated nodes of the other two) */
+int cbns_face_count(cbns_block* blk, int32_t axis, int64_t* n);
+/* Training samples from the plain FP64 solver state of buffer `buf`
+ * (CBNS_BUF_*): d_feat[n][32] features, d_label[n] = max |omega_k - d_k|
+ * over characteristic fields, sides and sub-stencils; sample order i fastest
+ * over the face box. */
+int cbns_face_samples(cbns_block* blk, int32_t axis, int32_t buf, float* d_feat, float* d_label);
+/* Load the indicator used by CBNS_FORM_HYBRID: a face is troubled when its
+ * logit exceeds `threshold`; troubled faces dilated by one cell along their
+ * axis run characteristic WENO5, the others the linear 5th-order upwind
+ * reconstruction.  options bit 0: evaluate the indicator at every stage
+ * (default: once per step, on the stage-1 input); bit 1: count troubled faces
+ * (cbns_flag_count). */
+int cbns_set_indicator(cbns_block* blk, const float* params, float threshold, int32_t options);
+/* troubled faces (before dilation) per axis found by the last indicator pass
+ * (needs options bit 1) */
+int cbns_flag_count(cbns_block* blk, int64_t counts[3]);
+/* the troubled-face bytes of the last indicator pass: 3 ghosted fields of
+ * uint8 (axis-major; face m of an axis, between nodes m and m+1, at node m),
+ * 3 * cbns_field_size bytes */
+int cbns_get_flags(cbns_block* blk, uint8_t* host);
+
+/* The network alone, for training on the GPU (forward, backward and Adam in
+ * HIP kernels on the bf16 matrix cores with fp32 accumulation). */
+typedef struct cbns_mlp cbns_mlp;
+int  cbns_mlp_create(cbns_mlp** out);
+void cbns_mlp_destroy(cbns_mlp* mlp);
+const char* cbns_mlp_last_error(const cbns_mlp* mlp);
+/* host arrays of CBNS_MLP_NPARAM floats; set also clears the Adam moments */
+int cbns_mlp_set_params(cbns_mlp* mlp, const float* params);
+int cbns_mlp_get_params(cbns_mlp* mlp, float* params);
+/* d_logit[n] = network(d_feat[n][32]) */
+int cbns_mlp_forward(cbns_mlp* mlp, const float* d_feat, int64_t n, float* d_logit);
+/* weighted logistic loss mean_i [t_i w_pos softplus(-s_i) + (1-t_i) softplus(s_i)]
+ * over targets d_target[n] in {0,1}; its gradient is kept for
+ * cbns_mlp_adam_step and, when `grad` (host, CBNS_MLP_NPARAM) is given, copied
+ * out. */
+int cbns_mlp_loss_grad(cbns_mlp* mlp, const float* d_feat, const float* d_target, int64_t n,
+                       float w_pos, double* loss, float* grad);
+/* one Adam step on the last gradient */
+int cbns_mlp_adam_step(cbns_mlp* mlp, float lr, float beta1, float beta2, float eps);
+
+/* Device buffers of this library's HIP runtime, for callers that have no
+ * HIP runtime of their own (or one from another build, as a Python process
+ * with a framework that bundles its own: device pointers must not cross
+ * runtimes).  All calls are synchronous on the current device. */
+int cbns_dev_alloc(int64_t bytes, void** d);
+int cbns_dev_free(void* d);
+int cbns_dev_upload(void* d, const void* host, int64_t bytes);
+int cbns_dev_download(void* host, const void* d, int64_t bytes);
+/* d_dst[r][0..row) = d_src[d_idx[r]][0..row) for r < n (float rows) */
+int cbns_gather_rows(const float* d_src, const int64_t* d_idx, int64_t n, int32_t row, float* d_dst);
 
 #ifdef __cplusplus
 }

@@ -425,8 +425,32 @@ void orc_conservative(const orc_grid* g, const double* prim, double* Q) {
  * flux of the face between node m and m+1 along direction d is stored at
  * flux[(d*5 + c)*S + idx(node m)].  Faces: 0..N-2 (reference axis) or -1..N-1
  * (periodic); transverse nodes: the update range. */
-void orc_numerical_flux(const orc_grid* g, const double* Q, const double* prim,
-                        const double* met, double* flux) {
+/* Face flux with the linear weights d = (0.1, 0.6, 0.3) (the learned hybrid's
+ * unflagged faces): since R L = I, R (linear(L q)) = linear(q), so the
+ * reconstruction runs on the conservative stencil; then primitives as in
+ * src/weno.cu:106-118 and the flux of src/lf_flux.cu. */
+void orc_face_flux_linear(const orc_grid* g, const double q[30], double nx, double ny, double nz,
+                          double flux[5]) {
+  double qs[2][5], pm[2][5];
+  for (int c = 0; c < 5; c++) {
+    qs[0][c] = (2.0 * q[0 * 5 + c] - 13.0 * q[1 * 5 + c] + 47.0 * q[2 * 5 + c] +
+                27.0 * q[3 * 5 + c] - 3.0 * q[4 * 5 + c]) / 60.0;
+    qs[1][c] = (2.0 * q[5 * 5 + c] - 13.0 * q[4 * 5 + c] + 47.0 * q[3 * 5 + c] +
+                27.0 * q[2 * 5 + c] - 3.0 * q[1 * 5 + c]) / 60.0;
+  }
+  for (int side = 0; side < 2; side++) {
+    const double rho = qs[side][0], u = qs[side][1] / rho, v = qs[side][2] / rho;
+    const double w = qs[side][3] / rho;
+    pm[side][0] = rho; pm[side][1] = u; pm[side][2] = v; pm[side][3] = w;
+    pm[side][4] = g->gam1 * (qs[side][4] - 0.5 * rho * (u * u + v * v + w * w));
+  }
+  orc_lf_flux(g, pm[1], pm[0], nx, ny, nz, flux);
+}
+
+/* flags: NULL, or 3*S troubled-face bytes (face m of axis d at d*S + node m);
+ * a face is WENO when it or a neighbour along its axis is troubled. */
+static void numerical_flux(const orc_grid* g, const double* Q, const double* prim,
+                           const double* met, const unsigned char* flags, double* flux) {
   const long S = g->S;
   for (int d = 0; d < 3; d++) {
     int lo[3], hi[3];
@@ -451,10 +475,24 @@ void orc_numerical_flux(const orc_grid* g, const double* Q, const double* prim,
           for (int c = 0; c < 5; c++) { pl[c] = prim[c * S + a]; pr[c] = prim[c * S + b]; }
           for (int s = 0; s < 6; s++)
             for (int c = 0; c < 5; c++) q[s * 5 + c] = Q[c * S + a + (s - 2) * st];
-          orc_face_flux(g, q, pl, pr, n_x, n_y, n_z, f);
+          const unsigned char* fl = flags ? flags + (long)d * S : NULL;
+          if (!fl || (fl[a - st] | fl[a] | fl[a + st]))
+            orc_face_flux(g, q, pl, pr, n_x, n_y, n_z, f);
+          else
+            orc_face_flux_linear(g, q, n_x, n_y, n_z, f);
           for (int c = 0; c < 5; c++) flux[(d * 5 + c) * S + a] = f[c];
         }
   }
+}
+
+void orc_numerical_flux(const orc_grid* g, const double* Q, const double* prim,
+                        const double* met, double* flux) {
+  numerical_flux(g, Q, prim, met, NULL, flux);
+}
+
+void orc_numerical_flux_hybrid(const orc_grid* g, const double* Q, const double* prim,
+                               const double* met, const unsigned char* flags, double* flux) {
+  numerical_flux(g, Q, prim, met, flags, flux);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -660,19 +698,31 @@ static void orc_sutherland(const orc_grid* g, const double* prim, double* mu) {
 
 /* One stage of src/Block3d.cpp:249-287.  Precondition: prim holds the
  * primitives of the stage input (Q for stage 1, Qp otherwise). */
-int orc_rk3_stage(const orc_grid* g, double dt, int stage, double* Q, double* Qp,
-                  double* prim, const double* met, double* mu) {
+static int rk3_stage(const orc_grid* g, double dt, int stage, double* Q, double* Qp,
+                     double* prim, const double* met, double* mu, const unsigned char* flags) {
   const long S = g->S;
   double* in = stage == 1 ? Q : Qp;
   double* flux = (double*)calloc((size_t)15 * S, sizeof(double));
   double* dfv = g->viscous ? (double*)calloc((size_t)4 * S, sizeof(double)) : NULL;
   if (!flux || (g->viscous && !dfv)) { free(flux); free(dfv); return -1; }
-  orc_numerical_flux(g, in, prim, met, flux);
+  numerical_flux(g, in, prim, met, flags, flux);
   if (g->viscous) orc_viscous(g, prim, met, mu, dfv);
   orc_update_rk3(g, dt, stage, met, flux, dfv, Q, Qp);
   orc_bc_primitive(g, stage == 3 ? Q : Qp, prim, met);
   free(flux); free(dfv);
   return 0;
+}
+
+int orc_rk3_stage(const orc_grid* g, double dt, int stage, double* Q, double* Qp,
+                  double* prim, const double* met, double* mu) {
+  return rk3_stage(g, dt, stage, Q, Qp, prim, met, mu, NULL);
+}
+
+/* the learned hybrid's stage with given troubled-face flags */
+int orc_rk3_stage_hybrid(const orc_grid* g, double dt, int stage, double* Q, double* Qp,
+                         double* prim, const double* met, double* mu,
+                         const unsigned char* flags) {
+  return rk3_stage(g, dt, stage, Q, Qp, prim, met, mu, flags);
 }
 
 /* src/Block3d.cpp:237-287 (one pass of the time loop) */

@@ -93,6 +93,31 @@ int  orc_rk3_step(const orc_grid* g, double t_end, double* t_cur, double* dt_out
 void orc_conservative(const orc_grid* g, const double* prim, double* Q);
 int  orc_threads(void);
 
+/* ---- learned troubled-face indicator (orc_learned.c) -------------------- */
+enum {
+  ORC_MLP_NFEAT = 30, ORC_MLP_IN = 32, ORC_MLP_HID = 32,
+  ORC_MLP_W1 = 0, ORC_MLP_B1 = 1024, ORC_MLP_W2 = 1056, ORC_MLP_B2 = 2080,
+  ORC_MLP_W3 = 2112, ORC_MLP_B3 = 2144, ORC_MLP_NPARAM = 2145
+};
+float  orc_bf16_round(float x);
+void   orc_face_features(const orc_grid* g, const double q[30], const double pl[5],
+                         const double pr[5], double nx, double ny, double nz, float feat[32]);
+void   orc_weno5_weights(double a, double b, double c, double d, double e, double om[3]);
+double orc_face_label(const orc_grid* g, const double q[30], const double pl[5],
+                      const double pr[5], double nx, double ny, double nz);
+void   orc_mlp_forward(const float* params, const float* X, long n, int bf16, float* logits);
+double orc_mlp_loss_grad(const float* params, const float* X, const float* y, long n,
+                         double w_pos, int bf16, double* grad);
+void   orc_face_flux_linear(const orc_grid* g, const double q[30], double nx, double ny,
+                            double nz, double flux[5]);
+void   orc_numerical_flux_hybrid(const orc_grid* g, const double* Q, const double* prim,
+                                 const double* met, const unsigned char* flags, double* flux);
+int    orc_rk3_stage_hybrid(const orc_grid* g, double dt, int stage, double* Q, double* Qp,
+                            double* prim, const double* met, double* mu,
+                            const unsigned char* flags);
+void   orc_adam(float* params, const double* grad, double* m, double* v, long np, double lr,
+                double b1, double b2, double eps, int t);
+
 #ifdef __cplusplus
 }
 #endif

@@ -83,6 +83,21 @@ def lib():
         L.orc_rk3_step.argtypes = [GP, C.c_double, D, D, D, D, D, D, D]
         L.orc_conservative.argtypes = [GP, D, D]
         L.orc_threads.restype = C.c_int
+        F = C.POINTER(C.c_float)
+        L.orc_bf16_round.argtypes = [C.c_float]
+        L.orc_bf16_round.restype = C.c_float
+        L.orc_face_features.argtypes = [GP, D, D, D, C.c_double, C.c_double, C.c_double, F]
+        L.orc_weno5_weights.argtypes = [C.c_double] * 5 + [D]
+        L.orc_face_label.argtypes = [GP, D, D, D, C.c_double, C.c_double, C.c_double]
+        L.orc_face_label.restype = C.c_double
+        L.orc_mlp_forward.argtypes = [F, F, C.c_long, C.c_int, F]
+        L.orc_mlp_loss_grad.argtypes = [F, F, F, C.c_long, C.c_double, C.c_int, D]
+        L.orc_mlp_loss_grad.restype = C.c_double
+        L.orc_face_flux_linear.argtypes = [GP, D, C.c_double, C.c_double, C.c_double, D]
+        U8 = C.c_void_p
+        L.orc_numerical_flux_hybrid.argtypes = [GP, D, D, D, U8, D]
+        L.orc_rk3_stage_hybrid.argtypes = [GP, C.c_double, C.c_int, D, D, D, D, D, U8]
+        L.orc_adam.argtypes = [F, D, D, D, C.c_long] + [C.c_double] * 4 + [C.c_int]
         _lib = L
     return _lib
 
@@ -162,6 +177,63 @@ class Oracle:
         if rc:
             raise RuntimeError("orc_rk3_stage failed")
 
+    def rk3_stage_hybrid(self, dt, stage, flags):
+        """stage with the learned hybrid's troubled-face bytes (3, *ghosted shape)"""
+        flags = np.ascontiguousarray(flags, dtype=np.uint8)
+        assert flags.size == 3 * self.g.S
+        rc = lib().orc_rk3_stage_hybrid(C.byref(self.g), dt, stage, ptr(self.Q), ptr(self.Qp),
+                                        ptr(self.prim), ptr(self.met), ptr(self.mu),
+                                        flags.ctypes.data)
+        if rc:
+            raise RuntimeError("orc_rk3_stage_hybrid failed")
+
+    def face_features_all(self, axis, Q=None):
+        """features (n, 32) and labels (n,) of every face the flux kernels
+        compute on `axis`, in the GPU library's sample order (i fastest)."""
+        Q = self.Q if Q is None else Q
+        g = self.g
+        lo, hi = [], []
+        for a, n in enumerate((g.IM, g.JM, g.KM)):
+            per = g.bc[a] == BC_PERIODIC
+            if a == axis:
+                lo.append(-1 if per else 0)
+                hi.append(n - 1 if per else n - 2)
+            else:
+                lo.append(0 if per else 1)
+                hi.append(n - 1 if per else n - 2)
+        feats, labels = [], []
+        L = lib()
+        f = np.zeros(32, dtype=np.float32)
+        q = np.zeros((6, 5))
+        pl = np.zeros(5)
+        pr = np.zeros(5)
+        for k in range(lo[2], hi[2] + 1):
+            for j in range(lo[1], hi[1] + 1):
+                for i in range(lo[0], hi[0] + 1):
+                    node = [i, j, k]
+                    for s in range(6):
+                        nn = list(node)
+                        nn[axis] += s - 2
+                        q[s] = Q[:, nn[2] + G, nn[1] + G, nn[0] + G]
+                    nb = list(node)
+                    nb[axis] += 1
+                    a_ = (slice(None), k + G, j + G, i + G)
+                    b_ = (slice(None), nb[2] + G, nb[1] + G, nb[0] + G)
+                    for side, (qq, out) in enumerate(((q[2], pl), (q[3], pr))):
+                        r = qq[0]
+                        u, v, w = qq[1] / r, qq[2] / r, qq[3] / r
+                        out[:] = (r, u, v, w, g.gam1 * (qq[4] - 0.5 * r * (u * u + v * v + w * w)))
+                    m = self.met
+                    n = [0.5 * (m[3 * axis + d][a_[1:]] * m[9][a_[1:]] +
+                                m[3 * axis + d][b_[1:]] * m[9][b_[1:]]) for d in range(3)]
+                    qf = np.ascontiguousarray(q.reshape(30))
+                    L.orc_face_features(C.byref(g), ptr(qf), ptr(pl), ptr(pr), n[0], n[1], n[2],
+                                        fptr(f))
+                    feats.append(f.copy())
+                    labels.append(L.orc_face_label(C.byref(g), ptr(qf), ptr(pl), ptr(pr),
+                                                   n[0], n[1], n[2]))
+        return np.array(feats), np.array(labels)
+
     def rk3_step(self, t_end, t_cur):
         tc = C.c_double(t_cur)
         dt = C.c_double(0.0)
@@ -224,3 +296,69 @@ def grid_for(IM=8, JM=8, KM=8, **kw):
     g = Grid()
     lib().orc_grid_init(C.byref(g), C.byref(make_params(IM, JM, KM, **kw)))
     return g
+
+
+# ---- learned troubled-face indicator (oracle/orc_learned.c) -------------
+MLP_NFEAT, MLP_IN, MLP_HID = 30, 32, 32
+MLP_W1, MLP_B1, MLP_W2, MLP_B2, MLP_W3, MLP_B3, MLP_NPARAM = 0, 1024, 1056, 2080, 2112, 2144, 2145
+
+
+def fptr(a):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"], "float32 C-contiguous array required"
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def bf16_round(x):
+    return np.array([lib().orc_bf16_round(float(v)) for v in np.ravel(x)], dtype=np.float32)
+
+
+def _face_args(q, pl, pr):
+    return (np.ascontiguousarray(q, dtype=np.float64).reshape(30),
+            np.ascontiguousarray(pl, dtype=np.float64), np.ascontiguousarray(pr, dtype=np.float64))
+
+
+def face_features(grid, q, pl, pr, n):
+    q, pl, pr = _face_args(q, pl, pr)
+    out = np.zeros(32, dtype=np.float32)
+    lib().orc_face_features(C.byref(grid), ptr(q), ptr(pl), ptr(pr), n[0], n[1], n[2], fptr(out))
+    return out
+
+
+def weno5_weights(a, b, c, d, e):
+    om = np.zeros(3)
+    lib().orc_weno5_weights(a, b, c, d, e, ptr(om))
+    return om
+
+
+def face_label(grid, q, pl, pr, n):
+    q, pl, pr = _face_args(q, pl, pr)
+    return lib().orc_face_label(C.byref(grid), ptr(q), ptr(pl), ptr(pr), n[0], n[1], n[2])
+
+
+def mlp_forward(params, X, bf16=True):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_IN)
+    out = np.zeros(len(X), dtype=np.float32)
+    lib().orc_mlp_forward(fptr(params), fptr(X), len(X), int(bf16), fptr(out))
+    return out
+
+
+def mlp_loss_grad(params, X, y, w_pos=1.0, bf16=True, grad=True):
+    params = np.ascontiguousarray(params, dtype=np.float32)
+    X = np.ascontiguousarray(X, dtype=np.float32).reshape(-1, MLP_IN)
+    y = np.ascontiguousarray(y, dtype=np.float32)
+    g = np.zeros(MLP_NPARAM) if grad else None
+    loss = lib().orc_mlp_loss_grad(fptr(params), fptr(X), fptr(y), len(X), w_pos, int(bf16),
+                                   ptr(g))
+    return (loss, g) if grad else loss
+
+
+def adam(params, grad, m, v, lr, b1=0.9, b2=0.999, eps=1e-8, t=1):
+    lib().orc_adam(fptr(params), ptr(grad), ptr(m), ptr(v), len(params), lr, b1, b2, eps, t)
+
+
+def face_flux_linear(grid, q, n):
+    out = np.zeros(5)
+    q = np.ascontiguousarray(q, dtype=np.float64).reshape(30)
+    lib().orc_face_flux_linear(C.byref(grid), ptr(q), n[0], n[1], n[2], ptr(out))
+    return out

@@ -47,6 +47,18 @@ def rel_err(a, b):
     return max(out)
 
 
+def rel_err_state(a, b):
+    """max |a-b| per conservative component over the scale of its quantity:
+    rho and E by their own max, the three momentum components by the largest
+    momentum magnitude (a component that is ~0 everywhere, like w of the
+    Taylor-Green vortex, is measured against the momentum scale)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    mom = max(np.abs(b[1:4]).max(), 1e-300)
+    scale = [max(np.abs(b[0]).max(), 1e-300), mom, mom, mom, max(np.abs(b[4]).max(), 1e-300)]
+    return max(float(np.abs(a[c] - b[c]).max() / scale[c]) for c in range(5))
+
+
 def update_mask(case):
     """Boolean mask (ghosted shape) of the RK-updated nodes."""
     p = case.params
