@@ -94,6 +94,72 @@ def cpu_baseline(N=128, steps=4):
                       % (N, steps, el)}
 
 
+def run_form(case, form, params, thresh, steps, warmup, profile=True):
+    """Time `steps` RK3 steps of one arithmetic form after `warmup` steps;
+    (seconds, per-kernel device times, all-finite)."""
+    from cbns3d_cuda_amd import cases
+    from cbns3d_cuda_amd.block import BUF_Q, F_Q, FORM_HYBRID, Block
+
+    b = Block(case.params)
+    try:
+        b.calc_metrics(case.x, case.y, case.z, case.period)
+        b.set_conservative(cases.q_ref_aos(case))
+        b.set_bc_calc_primitive(BUF_Q)
+        if form & FORM_HYBRID:
+            b.set_indicator(params, thresh)
+        b.set_form(form)
+        t = 0.0
+        for _ in range(warmup):
+            t, dt = b.rk3_step(1e9, t)
+        b.synchronize()
+        b.kernel_times(reset=True)
+        b.set_profiling(profile)
+        b.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            t, dt = b.rk3_step(1e9, t)
+        b.synchronize()
+        el = time.perf_counter() - t0
+        b.set_profiling(False)
+        kt = b.kernel_times(reset=True)
+        finite = bool(np.isfinite(b.get_field(F_Q)).all())
+    finally:
+        b.free()
+    return el, kt, finite
+
+
+def other_forms(case, args):
+    """FP32 and FP32 + learned hybrid: rates over the same timed steps, and
+    the hybrid's rho/E error against the FP64 form after `check_steps` steps
+    from the same initial state."""
+    from cbns3d_cuda_amd import learned
+    from cbns3d_cuda_amd.block import FORM_FP32, FORM_HYBRID
+
+    N = case.params.IM
+    out = {}
+    el, kt, finite = run_form(case, FORM_FP32, None, 0.0, args.steps, args.warmup, profile=False)
+    out["fp32"] = {"value": N ** 3 * args.steps / el / 1e6, "ms_per_step": 1e3 * el / args.steps,
+                   "finite": finite}
+    if not os.path.exists(args.indicator):
+        out["fp32_hybrid"] = {"skipped": "no indicator weights at %s" % args.indicator}
+        return out
+    P, thr, meta = learned.load(args.indicator)
+    form = FORM_FP32 | FORM_HYBRID
+    el, kt, finite = run_form(case, form, P, thr, args.steps, args.warmup)
+    h = {"value": N ** 3 * args.steps / el / 1e6, "ms_per_step": 1e3 * el / args.steps,
+         "finite": finite, "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in kt.items() if v[1]},
+         "indicator": os.path.basename(args.indicator), "threshold": thr}
+    if args.check_steps > 0:
+        log("bench: hybrid error check, %d steps" % args.check_steps)
+        err = learned.compare(case, args.check_steps, form, P, thr, log=log)
+        h.update({"check_steps": args.check_steps, "L2_rho": err["L2_rho"], "Linf_rho": err["Linf_rho"],
+                  "L2_E": err["L2_E"], "Linf_E": err["Linf_E"],
+                  "troubled_fraction_mean": err.get("troubled_fraction_mean"),
+                  "meets_L2_rho_1e-6": err["L2_rho"] <= 1e-6})
+    out["fp32_hybrid"] = h
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +169,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=128)
     ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--no-forms", action="store_true",
+                    help="measure only the FP64 plain form")
+    ap.add_argument("--indicator", default=os.path.join(ROOT, "cbns3d_cuda_amd",
+                                                        "indicator_weights.npz"))
+    ap.add_argument("--check-steps", type=int, default=200,
+                    help="steps of the hybrid's error check against FP64 (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; multi-GPU halo exchange is a design "
@@ -110,35 +182,11 @@ def main():
         sys.exit(2)
 
     from cbns3d_cuda_amd import cases
-    from cbns3d_cuda_amd.block import BUF_Q, F_Q, Block
 
     N = args.size
     log("bench: building taylor_green %d^3" % N)
     case = cases.taylor_green(N)
-    b = Block(case.params)
-    b.calc_metrics(case.x, case.y, case.z, case.period)
-    b.set_conservative(cases.q_ref_aos(case))
-    b.set_bc_calc_primitive(BUF_Q)
-    del case
-
-    t = 0.0
-    for _ in range(args.warmup):
-        t, dt = b.rk3_step(1e9, t)
-    b.synchronize()
-    b.kernel_times(reset=True)
-    b.set_profiling(True)
-    b.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        t, dt = b.rk3_step(1e9, t)
-    b.synchronize()
-    el = time.perf_counter() - t0
-    b.set_profiling(False)
-    kt = b.kernel_times(reset=True)
-    q0 = b.get_field(F_Q)
-    finite = bool(np.isfinite(q0).all())
-    b.free()
-
+    el, kt, finite = run_form(case, 0, None, 0.0, args.steps, args.warmup)
     value = N ** 3 * args.steps / el / 1e6
     log("bench: %.3f ms/step, %.1f Mcell-RK3-steps/s, finite=%s" % (1e3 * el / args.steps, value, finite))
     for k, (ms, n) in kt.items():
@@ -151,6 +199,13 @@ def main():
     dom = max(kt, key=lambda k: kt[k][0])
     roof = roofline_of(dom, kt[dom][0], kt[dom][1], N)
     per_kernel = {k: roofline_of(k, ms, n, N) for k, (ms, n) in kt.items() if n}
+
+    forms = None
+    if not args.no_forms:
+        forms = other_forms(case, args)
+        for k, v in forms.items():
+            log("  form %s: %s" % (k, json.dumps(v)))
+    del case
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -175,6 +230,7 @@ def main():
                    "form": "fp64-plain", "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "forms": forms,
         "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in kt.items() if v[1]},
         "kernel_roofline": {k: {"frac": r["frac"], "achieved": r["achieved"], "unit": r["unit"]}
                             for k, r in per_kernel.items()},

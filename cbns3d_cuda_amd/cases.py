@@ -16,6 +16,12 @@ initial state at the nodes.  Nothing here runs per step.
                      (far field on xi, wake cut / wall on eta = 0, frozen far
                      field on eta = JM-1, copies on zeta) with a perturbed
                      freestream
+  shock_vortex       Mach-2 normal shock hit by a compressible columnar vortex
+                     (strength 0.3, core radius 0.05), inviscid, on a
+                     [0,2]x[0,1]x[0,1] mesh clustered (spacing ratio 1.5) at
+                     the shock plane and gently curved; xi takes the
+                     reference's far-field rule, eta and zeta are periodic;
+                     vortex centre and sense of rotation from a seeded PCG64
 """
 from dataclasses import dataclass
 
@@ -110,6 +116,81 @@ def c_grid_channel(IM=24, JM=16, KM=8, *, viscous=0, seed=7, Mach=0.5, CFL=0.5,
     return Case("c_grid_channel", params, x, y, z, None, np.stack([rho, u, v, w, p]))
 
 
+def _clustered(n, length, x_s, ratio=1.5, width=0.05):
+    """Node positions of n points on [0, length) whose spacing is `ratio`
+    times finer at x_s than far from it: dx/ds ~ 1 - k sech^2((s - s_s)/w)
+    with k = 1 - 1/ratio, integrated in closed form; s_s solves x(s_s) = x_s."""
+    k = 1.0 - 1.0 / ratio
+    s = np.arange(n, dtype=np.float64) / n
+
+    def raw(t, ss):
+        return t - k * width * (np.tanh((t - ss) / width) - np.tanh(-ss / width))
+
+    def mapped(t, ss):
+        return length * raw(t, ss) / raw(1.0, ss)
+
+    lo, hi = 0.0, 1.0
+    for _ in range(200):  # bisection on the monotone map s_s -> x(s_s)
+        mid = 0.5 * (lo + hi)
+        if mapped(mid, mid) < x_s:
+            lo = mid
+        else:
+            hi = mid
+    return mapped(s, 0.5 * (lo + hi))
+
+
+def shock_vortex(IM=256, JM=128, KM=128, *, seed=0, CFL=0.5, Mach=2.0, gamma=1.4,
+                 strength=0.3, core=0.05, x_shock=0.6):
+    """Mach-2 stationary normal shock at x_shock with a columnar (z-aligned)
+    vortex upstream, moving with the freestream into it.  Vortex:
+    u_theta = s eps (r/rc) exp((1 - (r/rc)^2)/2), eps = strength (freestream
+    speed 1), isentropic core T = 1 - (gamma-1) M^2 eps^2 exp(1 - (r/rc)^2)/2.
+    Downstream: the Rankine-Hugoniot state of the freestream."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    xv = rng.uniform(0.25, 0.35)
+    yv = rng.uniform(0.35, 0.65)
+    sense = 1.0 if rng.uniform() < 0.5 else -1.0
+    Lx, Ly, Lz = 2.0, 1.0, 1.0
+    xs = _clustered(IM, Lx, x_shock)
+    ys = _node_coords(JM, Ly)
+    zs = _node_coords(KM, Lz)
+    zeta, eta, xi = np.meshgrid(zs, ys, xs, indexing="ij")
+    bump = np.sin(np.pi * xi / Lx)
+    a = 0.01
+    x = xi + a * np.sin(2 * np.pi * eta / Ly) * np.sin(2 * np.pi * zeta / Lz)
+    y = eta + a * np.sin(2 * np.pi * zeta / Lz) * bump
+    z = zeta + a * np.sin(2 * np.pi * eta / Ly) * bump
+    # freestream (rho = 1, |u| = 1, p = 1/(gamma M^2)) plus the vortex
+    dx, dy = x - xv, y - yv
+    r2 = (dx * dx + dy * dy) / (core * core)
+    g = np.exp(0.5 * (1.0 - r2))
+    u = 1.0 - sense * strength * (dy / core) * g
+    v = sense * strength * (dx / core) * g
+    w = np.zeros_like(u)
+    T = 1.0 - 0.5 * (gamma - 1.0) * Mach ** 2 * strength ** 2 * g * g
+    rho = T ** (1.0 / (gamma - 1.0))
+    p = rho * T / (gamma * Mach ** 2)
+    # Rankine-Hugoniot jump for the freestream Mach number
+    M2 = Mach * Mach
+    r_ratio = (gamma + 1.0) * M2 / ((gamma - 1.0) * M2 + 2.0)
+    p_ratio = 1.0 + 2.0 * gamma / (gamma + 1.0) * (M2 - 1.0)
+    down = x > x_shock
+    rho = np.where(down, r_ratio, rho)
+    u = np.where(down, 1.0 / r_ratio, u)
+    v = np.where(down, 0.0, v)
+    p = np.where(down, p_ratio / (gamma * M2), p)
+    params = make_params(IM, JM, KM, viscous=0, bc=(BC_REF, BC_PERIODIC, BC_PERIODIC), CFL=CFL,
+                         gamma=gamma, Mach=Mach)
+    period = np.diag([0.0, Ly, Lz])
+    return Case("shock_vortex_%dx%dx%d_s%d" % (IM, JM, KM, seed), params, x, y, z, period,
+                np.stack([rho, u, v, w, p]))
+
+
+def pad_modes(params):
+    """np.pad mode per node axis (K, J, I): wrap on periodic axes, edge else."""
+    return ["wrap" if params.bc[a] == BC_PERIODIC else "edge" for a in (2, 1, 0)]
+
+
 def conservative(prim, gamma):
     """(5, ...) primitives -> (5, ...) conservative (src/conservative_kernel.cu:42-46)."""
     rho, u, v, w, p = prim
@@ -118,13 +199,19 @@ def conservative(prim, gamma):
 
 
 def ghosted(arr, ng, mode="edge"):
-    """Pad node arrays (..., K, J, I) with ng ghost layers on each side."""
-    pad = [(0, 0)] * (arr.ndim - 3) + [(ng, ng)] * 3
-    return np.pad(arr, pad, mode=mode)
+    """Pad node arrays (..., K, J, I) with ng ghost layers on each side; mode
+    is one np.pad mode or a list of three (for K, J, I)."""
+    modes = [mode] * 3 if isinstance(mode, str) else list(mode)
+    out = arr
+    for ax, m in zip(range(arr.ndim - 3, arr.ndim), modes):
+        pad = [(0, 0)] * arr.ndim
+        pad[ax] = (ng, ng)
+        out = np.pad(out, pad, mode=m)
+    return out
 
 
 def q_ref_aos(case):
     """Conservative state in the reference's host layout: AoS, 2 ghost layers."""
     q = conservative(case.prim, case.params.gamma)
-    qg = ghosted(q, 2, mode="wrap" if case.period is not None else "edge")
+    qg = ghosted(q, 2, mode=pad_modes(case.params) if case.period is not None else "edge")
     return np.ascontiguousarray(np.moveaxis(qg, 0, -1))

@@ -100,7 +100,7 @@ __device__ inline bool box_node3(const Box& b, int& i, int& j, int& k) {
 // XCD's L2).  The grid is rounded up to a multiple of 8 blocks; blocks past
 // the tile count return.
 #ifndef CBNS_N_XCD
-#define CBNS_N_XCD 8
+#define CBNS_N_XCD 1
 #endif
 constexpr int N_XCD = CBNS_N_XCD;
 struct TileMarch {

@@ -21,7 +21,7 @@ namespace cbns {
 namespace {
 
 #ifndef VISC_CHUNK
-#define VISC_CHUNK 16  // nodes along k per thread
+#define VISC_CHUNK 1  // nodes along k per thread
 #endif
 
 // 4th-order derivative along axis a at node m of the viscous range:

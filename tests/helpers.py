@@ -18,7 +18,7 @@ def ghosted_q(case):
     """Full ghosted SoA conservative state: nodes from the case, ghosts by
     wrap (periodic) or edge replication (so every ghost holds a valid state)."""
     q = cases.conservative(case.prim, case.params.gamma)
-    mode = "wrap" if case.period is not None else "edge"
+    mode = cases.pad_modes(case.params) if case.period is not None else "edge"
     return np.ascontiguousarray(cases.ghosted(q, G, mode=mode))
 
 
