@@ -12,8 +12,11 @@
 //                  nodes, at the reference's transverse offset (-1) or
 //                  centred; written to dfv (reference decomposition) or
 //                  subtracted from the fused stage's right-hand side.
-// Both walk along k in chunks on XCD-contiguous tiles (TileMarch) so the
-// +-2 row and plane neighbours of the stencils are re-read from L2, not HBM.
+// Both walk along k in chunks (TileMarch): the five zeta-planes a 4th-order
+// stencil needs (u,v,w,T in the flux pass, Gv in the divergence) stay in a
+// register window, so each plane value is loaded once per chunk instead of
+// five times (a plane of one field is 0.56 MB at 256^3: the +-2 planes of
+// four fields do not fit a 4 MB XCD L2 and would be re-read from HBM).
 #include "cbns_internal.hpp"
 
 namespace cbns {
@@ -21,7 +24,7 @@ namespace cbns {
 namespace {
 
 #ifndef VISC_CHUNK
-#define VISC_CHUNK 1  // nodes along k per thread
+#define VISC_CHUNK 8  // nodes along k per thread
 #endif
 
 // 4th-order derivative along axis a at node m of the viscous range:
@@ -48,11 +51,18 @@ __device__ __forceinline__ double dgrad(const Consts& c, int a, int m, const dou
          1.0 / 12.0 * f[2 * st];
 }
 
+// 4th-order central difference from a 5-plane window (same arithmetic as
+// dgrad's central branch)
+__device__ __forceinline__ double dwin(const double w[5]) {
+  return (1.0 / 12.0) * w[0] - 2.0 / 3.0 * w[1] + (2.0 / 3.0) * w[3] - 1.0 / 12.0 * w[4];
+}
+
 // T = double (plain form) or float (precision form: the index derivatives
 // are differenced in double and rounded, stresses, heat flux and fluxes are
-// formed and stored in float).
+// formed and stored in float).  win[f][s]: field f (u,v,w,T) at k-2+s.
 template <class T>
 __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, int k,
+                                                 const double win[4][5],
                                                  const double* __restrict__ prim,
                                                  const double* __restrict__ met,
                                                  double* __restrict__ mu,
@@ -64,13 +74,20 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
   const double* W = prim + P_W * S + o;
   const double* TT = prim + P_T * S + o;
   const int sx = 1, sy = c.PX, sz = c.PXY;
-  T u_xi_l = (T)dgrad(c, 0, i, U, sx), u_eta_l = (T)dgrad(c, 1, j, U, sy), u_zeta_l = (T)dgrad(c, 2, k, U, sz);
-  const T v_xi_l = (T)dgrad(c, 0, i, V, sx), v_eta_l = (T)dgrad(c, 1, j, V, sy), v_zeta_l = (T)dgrad(c, 2, k, V, sz);
-  const T w_xi_l = (T)dgrad(c, 0, i, W, sx), w_eta_l = (T)dgrad(c, 1, j, W, sy), w_zeta_l = (T)dgrad(c, 2, k, W, sz);
-  const T T_xi = (T)dgrad(c, 0, i, TT, sx), T_eta = (T)dgrad(c, 1, j, TT, sy), T_zeta = (T)dgrad(c, 2, k, TT, sz);
+  // zeta: the register window, except at the one-sided outer layers of a
+  // reference axis
+  const bool zc = periodic(c, 2) || (k >= 0 && k <= c.KM - 1);
+  const double uz = zc ? dwin(win[0]) : dgrad(c, 2, k, U, sz);
+  const double vz = zc ? dwin(win[1]) : dgrad(c, 2, k, V, sz);
+  const double wz = zc ? dwin(win[2]) : dgrad(c, 2, k, W, sz);
+  const double tz = zc ? dwin(win[3]) : dgrad(c, 2, k, TT, sz);
+  T u_xi_l = (T)dgrad(c, 0, i, U, sx), u_eta_l = (T)dgrad(c, 1, j, U, sy), u_zeta_l = (T)uz;
+  const T v_xi_l = (T)dgrad(c, 0, i, V, sx), v_eta_l = (T)dgrad(c, 1, j, V, sy), v_zeta_l = (T)vz;
+  const T w_xi_l = (T)dgrad(c, 0, i, W, sx), w_eta_l = (T)dgrad(c, 1, j, W, sy), w_zeta_l = (T)wz;
+  const T T_xi = (T)dgrad(c, 0, i, TT, sx), T_eta = (T)dgrad(c, 1, j, TT, sy), T_zeta = (T)tz;
 
   // src/viscous_terms_kernel.cu:59-105
-  const double T_l = TT[0];
+  const double T_l = win[3][2];
   const double C_T_inf = c.C_T_inf;
   const double mu_d = (1.0 + C_T_inf) / (T_l + C_T_inf) * sqrt(T_l * T_l * T_l);
   if (mu) mu[o] = mu_d;
@@ -93,7 +110,7 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
 
   // src/viscous_flux_kernel.cu:57-110
   const double Jac_a = met[JAC * S + o];
-  const T uu = (T)U[0], vv = (T)V[0], ww = (T)W[0];
+  const T uu = (T)win[0][2], vv = (T)win[1][2], ww = (T)win[2][2];
   const T ex = uu * txx + vv * txy + ww * txz - qx;
   const T ey = uu * txy + vv * tyy + ww * tyz - qy;
   const T ez = uu * txz + vv * tyz + ww * tzz - qz;
@@ -118,30 +135,46 @@ __global__ void __launch_bounds__(256) k_visc_fluxes(Consts c, TileMarch tm,
                                                      T* __restrict__ vf) {
   int i, j, k0, k1;
   if (!tile_march_node(tm, i, j, k0, k1)) return;
-  for (int k = k0; k < k1; k++) visc_fluxes_node<T>(c, i, j, k, prim, met, mu, vf);
+  const long S = c.S;
+  const int sz = c.PXY;
+  const double* f0[4] = {prim + P_U * S, prim + P_V * S, prim + P_W * S, prim + P_T * S};
+  const int base = idx(c, i, j, 0);
+  double win[4][5];
+#pragma unroll
+  for (int f = 0; f < 4; f++)
+#pragma unroll
+    for (int s = 0; s < 4; s++) win[f][s + 1] = f0[f][base + (k0 - 2 + s) * sz];
+  for (int k = k0; k < k1; k++) {
+#pragma unroll
+    for (int f = 0; f < 4; f++) {
+#pragma unroll
+      for (int s = 0; s < 4; s++) win[f][s] = win[f][s + 1];
+      win[f][4] = f0[f][base + (k + 2) * sz];
+    }
+    visc_fluxes_node<T>(c, i, j, k, win, prim, met, mu, vf);
+  }
 }
 
 // src/viscous_contribution_kernel.cu:35-67.  mode 0: dfv[e] = div; mode 1:
 // out[(e+1)] -= div (the fused stage's right-hand side, momentum and energy).
+// gw[e][s]: Gv component e on the zeta line of the stencil at k-2+s.
 template <class T>
 __device__ __forceinline__ void visc_div_node(const Consts& c, int i, int j, int k,
-                                              const T* __restrict__ vf,
+                                              const T gw[4][5], const T* __restrict__ vf,
                                               double* __restrict__ out, int mode) {
   const long S = c.S;
   const int sh = c.visc_centered ? 0 : -1;
   const T c0 = T(1.0 / 12.0), c1 = T(-2.0 / 3.0);
-  const int ox = idx(c, i, j + sh, k + sh), oy = idx(c, i + sh, j, k + sh),
-            oz = idx(c, i + sh, j + sh, k);
-  const int sx = 1, sy = c.PX, sz = c.PXY;
+  const int ox = idx(c, i, j + sh, k + sh), oy = idx(c, i + sh, j, k + sh);
+  const int sx = 1, sy = c.PX;
   const int o = idx(c, i, j, k);
 #pragma unroll
   for (int e = 0; e < 4; e++) {
     const T* Ev = vf + (long)e * S;
     const T* Fv = vf + (long)(4 + e) * S;
-    const T* Gv = vf + (long)(8 + e) * S;
     T df = c0 * Ev[ox - 2 * sx] + c1 * Ev[ox - sx] - c1 * Ev[ox + sx] - c0 * Ev[ox + 2 * sx];
     df += c0 * Fv[oy - 2 * sy] + c1 * Fv[oy - sy] - c1 * Fv[oy + sy] - c0 * Fv[oy + 2 * sy];
-    df += c0 * Gv[oz - 2 * sz] + c1 * Gv[oz - sz] - c1 * Gv[oz + sz] - c0 * Gv[oz + 2 * sz];
+    df += c0 * gw[e][0] + c1 * gw[e][1] - c1 * gw[e][3] - c0 * gw[e][4];
     if (mode == 0) out[e * S + o] = (double)df;
     else out[(e + 1) * S + o] -= (double)df;
   }
@@ -153,7 +186,25 @@ __global__ void __launch_bounds__(256) k_visc_div(Consts c, TileMarch tm,
                                                   double* __restrict__ out, int mode) {
   int i, j, k0, k1;
   if (!tile_march_node(tm, i, j, k0, k1)) return;
-  for (int k = k0; k < k1; k++) visc_div_node<T>(c, i, j, k, vf, out, mode);
+  const long S = c.S;
+  const int sh = c.visc_centered ? 0 : -1;
+  const int sz = c.PXY;
+  const int base = idx(c, i + sh, j + sh, 0);  // the zeta line of the Gv stencil
+  const T* G0 = vf + 8 * S;
+  T gw[4][5];
+#pragma unroll
+  for (int e = 0; e < 4; e++)
+#pragma unroll
+    for (int s = 0; s < 4; s++) gw[e][s + 1] = G0[e * S + base + (k0 - 2 + s) * sz];
+  for (int k = k0; k < k1; k++) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+#pragma unroll
+      for (int s = 0; s < 4; s++) gw[e][s] = gw[e][s + 1];
+      gw[e][4] = G0[e * S + base + (k + 2) * sz];
+    }
+    visc_div_node<T>(c, i, j, k, gw, vf, out, mode);
+  }
 }
 
 }  // namespace
