@@ -1,11 +1,14 @@
 // k_fused.hip -- one RK3 stage as sweep kernels (cbns_rk3_stage):
 //
-//   k_xfaces     one thread per xi-face, lanes along i (coalesced); the face
-//                flux goes to HBM (5 doubles per face)
 //   k_march<1>   lanes along i, each thread marches a chunk of nodes along
 //                eta holding the 6-node stencil and the previous face flux in
 //                registers: each node is loaded once per sweep and each face
-//                is computed once.  Writes rhs = dE + dF.
+//                is computed once.  At every node it also computes the
+//                xi-face on its right; the left one comes from the lane below
+//                by a wavefront shuffle, so a wavefront updates 63 nodes and
+//                its lane 0 only supplies that face.  Writes rhs = dE + dF.
+//                (FUSE_XI=0: a separate k_xfaces pass writes the xi-face
+//                fluxes to HBM and the march reads them back.)
 //   [viscous]    k_visc_fluxes + k_visc_div subtract the viscous divergence
 //                from rhs (k_viscous.hip)
 //   k_march<2>   the same march along zeta; adds dG, divides by J and
@@ -35,6 +38,9 @@ constexpr int CHUNK = MARCH_CHUNK;  // nodes per thread along the march axis
 #endif
 #ifndef WPE_M
 #define WPE_M 2  // waves per SIMD the march kernels are compiled for
+#endif
+#ifndef FUSE_XI
+#define FUSE_XI 1  // xi-faces inside the eta march
 #endif
 
 template <class T>
@@ -94,14 +100,19 @@ struct StageArgs {
   double* rhs;          // dE + dF (+ viscous)
   const double* dfv;    // viscous divergence or nullptr
   const uint8_t* flags; // troubled faces of the march axis (HYB)
+  const uint8_t* flags_x;  // troubled xi-faces (HYB, fused xi-faces)
 };
 
 // AX = 1: march along eta, rhs = dE + dF.  AX = 2: march along zeta, finish.
 template <int AX, class F, bool HYB>
 __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves_per_eu(WPE_M, WPE_M))) k_march(Consts c, StageArgs sa,
                                                             const double* __restrict__ met) {
+  constexpr bool XI = AX == 1 && FUSE_XI;
   const int ax_t = AX == 1 ? 2 : 1;  // transverse axis other than i
-  const int i = upd_lo(c, 0) + blockIdx.x * MARCH_X + threadIdx.x;
+  const int lane = threadIdx.x;
+  // XI: a wavefront covers 63 nodes; lane 0 sits one node to the left
+  const int i = XI ? upd_lo(c, 0) - 1 + blockIdx.x * (MARCH_X - 1) + lane
+                   : upd_lo(c, 0) + blockIdx.x * MARCH_X + lane;
   const int t = upd_lo(c, ax_t) + blockIdx.y * MARCH_Y + threadIdx.y;
   if (i > upd_hi(c, 0) || t > upd_hi(c, ax_t)) return;
   const int mlo = upd_lo(c, AX), mhi = upd_hi(c, AX);
@@ -148,14 +159,40 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
     ax = bx; ay = by; az = bz;
     jb = J[o + st];
     bx = nX[o + st] * jb; by = nY[o + st] * jb; bz = nZ[o + st] * jb;
-    double f[5];
+    double f[5], fxr[5];
+    if (XI) {
+      // the xi-face between node (i, m) and (i+1, m): stencil i-2 .. i+3
+      double qx[6][5];
+#pragma unroll
+      for (int s = 0; s < 6; s++) {
+        if (s == 2) {
+#pragma unroll
+          for (int n = 0; n < 5; n++) qx[2][n] = q[2][n];
+        } else {
+          load_q(Q, S, o + (s - 2), qx[s]);
+        }
+      }
+      const double Ja = J[o], Jb = J[o + 1];
+      const double* __restrict__ xm = met + XI_X * S;
+      const double nxx = 0.5 * (xm[o] * Ja + xm[o + 1] * Jb);
+      const double nxy = 0.5 * (xm[S + o] * Ja + xm[S + o + 1] * Jb);
+      const double nxz = 0.5 * (xm[2 * S + o] * Ja + xm[2 * S + o + 1] * Jb);
+      const bool tx = HYB && (sa.flags_x[o - 1] | sa.flags_x[o] | sa.flags_x[o + 1]);
+      stage_face<F, HYB>(g, qx, nxx, nxy, nxz, tx, fxr);
+    }
     if (HYB) {
       t0 = t1;
       t1 = t2;
       t2 = fl[o + st];
     }
     stage_face<F, HYB>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), t0 | t1 | t2, f);
-    if (AX == 1) {
+    if (XI) {
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        const double fxl = __shfl_up(fxr[n], 1);
+        if (lane > 0) sa.rhs[n * S + o] = (fxr[n] - fxl) + (f[n] - fprev[n]);
+      }
+    } else if (AX == 1) {
 #pragma unroll
       for (int n = 0; n < 5; n++) {
         const double* fxn = sa.fx + n * S;
@@ -206,7 +243,7 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
   }
   const long S = c.S;
   const uint8_t* fl = HYB ? b->flags : nullptr;
-  {
+  if (!FUSE_XI) {
     Box box;
     for (int a = 0; a < 3; a++) {
       box.lo[a] = upd_lo(c, a);
@@ -218,12 +255,14 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
     k_xfaces<F, HYB><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fl, fx);
     CBNS_HIP(b, hipGetLastError());
   }
-  StageArgs sa{dt, stage, b->prim, b->Q, qin, qout, fx, b->rhs, nullptr, HYB ? fl + S : nullptr};
+  StageArgs sa{dt, stage, b->prim, b->Q, qin, qout, fx, b->rhs, nullptr,
+               HYB ? fl + S : nullptr, HYB ? fl : nullptr};
   const int ni = upd_hi(c, 0) - upd_lo(c, 0) + 1;
   {
     const int nt = upd_hi(c, 2) - upd_lo(c, 2) + 1;
     const int nm = upd_hi(c, 1) - upd_lo(c, 1) + 1;
-    dim3 grid((ni + MARCH_X - 1) / MARCH_X, (nt + MARCH_Y - 1) / MARCH_Y,
+    const int per_wave = FUSE_XI ? MARCH_X - 1 : MARCH_X;
+    dim3 grid((ni + per_wave - 1) / per_wave, (nt + MARCH_Y - 1) / MARCH_Y,
               (nm + CHUNK - 1) / CHUNK);
     ProfScope ps(b, CBNS_K_MARCH_ETA);
     k_march<1, F, HYB><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);

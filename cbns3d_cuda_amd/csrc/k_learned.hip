@@ -160,14 +160,15 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpDev w, const float* __restr
     a2[t] = w.W2A[t * 64 + l];
     a2t[t] = w.W2TA[t * 64 + l];
   }
-  float b1[2][4], b2[2][4], w3[2][4];
+  f32x4 cb1[2], cb2[2];
+  float w3[2][4];
 #pragma unroll
   for (int ht = 0; ht < 2; ht++)
 #pragma unroll
     for (int v = 0; v < 4; v++) {
       const int r = 16 * ht + 4 * lq + v;
-      b1[ht][v] = w.P[MLP_B1 + r];
-      b2[ht][v] = w.P[MLP_B2 + r];
+      cb1[ht][v] = w.P[MLP_B1 + r];
+      cb2[ht][v] = w.P[MLP_B2 + r];
       w3[ht][v] = w.P[MLP_W3 + r];
     }
   const float b3 = w.P[MLP_B3];
@@ -194,26 +195,21 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpDev w, const float* __restr
       bf16x4 h[2];
 #pragma unroll
       for (int ht = 0; ht < 2; ht++) {
-        z[ht] = mfma_bf16(a1[2 * ht], x[0], zero);
+        z[ht] = mfma_bf16(a1[2 * ht], x[0], cb1[ht]);  // biases seed the accumulators
         z[ht] = mfma_bf16(a1[2 * ht + 1], x[1], z[ht]);
 #pragma unroll
-        for (int v = 0; v < 4; v++) {
-          z[ht][v] += b1[ht][v];
-          h[ht][v] = f32_to_bf16(fmaxf(z[ht][v], 0.0f));
-        }
+        for (int v = 0; v < 4; v++) h[ht][v] = f32_to_bf16(relu(z[ht][v]));
       }
 #pragma unroll
       for (int ht = 0; ht < 2; ht++) {
-        y[ht] = mfma_bf16(a2[2 * ht], h[0], zero);
+        y[ht] = mfma_bf16(a2[2 * ht], h[0], cb2[ht]);
         y[ht] = mfma_bf16(a2[2 * ht + 1], h[1], y[ht]);
-#pragma unroll
-        for (int v = 0; v < 4; v++) y[ht][v] += b2[ht][v];
       }
       float part = 0.0f;
 #pragma unroll
       for (int ht = 0; ht < 2; ht++)
 #pragma unroll
-        for (int v = 0; v < 4; v++) part += w3[ht][v] * fmaxf(y[ht][v], 0.0f);
+        for (int v = 0; v < 4; v++) part += w3[ht][v] * relu(y[ht][v]);
       part += __shfl_xor(part, 16);
       part += __shfl_xor(part, 32);
       const float s = part + b3;
@@ -231,7 +227,7 @@ __global__ void __launch_bounds__(256) k_mlp_grad(MlpDev w, const float* __restr
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           const float act = y[ht][v] > 0.0f;
-          gw3[ht][v] += ds * fmaxf(y[ht][v], 0.0f);
+          gw3[ht][v] += ds * relu(y[ht][v]);
           const float dz2 = act ? ds * w3[ht][v] : 0.0f;
           gb2[ht][v] += dz2;
           dz2b[ht][v] = f32_to_bf16(dz2);

@@ -49,10 +49,15 @@ struct MlpDev {
   const float* P;      // fp32 parameters (biases, output layer)
 };
 
+// round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
 __device__ __forceinline__ short f32_to_bf16(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);  // round to nearest even
-  return (short)(u >> 16);
+  return __builtin_bit_cast(short, (__bf16)x);
+}
+
+// max(x, 0) on the bit pattern: negative floats (and -0) are negative
+// integers, so one integer max is the ReLU
+__device__ __forceinline__ float relu(float x) {
+  return __int_as_float(max(__float_as_int(x), 0));
 }
 
 __device__ __forceinline__ f32x4 mfma_bf16(bf16x4 a, bf16x4 b, f32x4 c) {
@@ -182,16 +187,17 @@ __device__ __forceinline__ float mlp_logit_wave(const float x[MLP_IN], const Mlp
   const bf16x4 a12 = w.W1A[2 * 64 + l], a13 = w.W1A[3 * 64 + l];
   const bf16x4 a20 = w.W2A[0 * 64 + l], a21 = w.W2A[1 * 64 + l];
   const bf16x4 a22 = w.W2A[2 * 64 + l], a23 = w.W2A[3 * 64 + l];
-  float b1[2][4], b2[2][4], w3[2][4];
+  f32x4 c10, c11, c20, c21;  // bias rows 4 (l / 16) + v of the two 16-row tiles
+  float w3[2][4];
 #pragma unroll
-  for (int ht = 0; ht < 2; ht++)
-#pragma unroll
-    for (int v = 0; v < 4; v++) {
-      const int r = 16 * ht + 4 * lq + v;
-      b1[ht][v] = w.P[MLP_B1 + r];
-      b2[ht][v] = w.P[MLP_B2 + r];
-      w3[ht][v] = w.P[MLP_W3 + r];
-    }
+  for (int v = 0; v < 4; v++) {
+    c10[v] = w.P[MLP_B1 + 4 * lq + v];
+    c11[v] = w.P[MLP_B1 + 16 + 4 * lq + v];
+    c20[v] = w.P[MLP_B2 + 4 * lq + v];
+    c21[v] = w.P[MLP_B2 + 16 + 4 * lq + v];
+    w3[0][v] = w.P[MLP_W3 + 4 * lq + v];
+    w3[1][v] = w.P[MLP_W3 + 16 + 4 * lq + v];
+  }
   const float b3 = w.P[MLP_B3];
   float mine = 0.0f;
 #pragma unroll
@@ -199,25 +205,25 @@ __device__ __forceinline__ float mlp_logit_wave(const float x[MLP_IN], const Mlp
     const short* xr = lds + (16 * ft + lr) * MLP_IN + 4 * lq;
     const bf16x4 x0 = *reinterpret_cast<const bf16x4*>(xr);
     const bf16x4 x1 = *reinterpret_cast<const bf16x4*>(xr + 16);
-    const f32x4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-    f32x4 z0 = mfma_bf16(a10, x0, zero);
+    // the biases seed the accumulators (D = A B + C)
+    f32x4 z0 = mfma_bf16(a10, x0, c10);
     z0 = mfma_bf16(a11, x1, z0);
-    f32x4 z1 = mfma_bf16(a12, x0, zero);
+    f32x4 z1 = mfma_bf16(a12, x0, c11);
     z1 = mfma_bf16(a13, x1, z1);
     bf16x4 h0, h1;
 #pragma unroll
     for (int v = 0; v < 4; v++) {
-      h0[v] = f32_to_bf16(fmaxf(z0[v] + b1[0][v], 0.0f));
-      h1[v] = f32_to_bf16(fmaxf(z1[v] + b1[1][v], 0.0f));
+      h0[v] = f32_to_bf16(relu(z0[v]));
+      h1[v] = f32_to_bf16(relu(z1[v]));
     }
-    f32x4 y0 = mfma_bf16(a20, h0, zero);
+    f32x4 y0 = mfma_bf16(a20, h0, c20);
     y0 = mfma_bf16(a21, h1, y0);
-    f32x4 y1 = mfma_bf16(a22, h0, zero);
+    f32x4 y1 = mfma_bf16(a22, h0, c21);
     y1 = mfma_bf16(a23, h1, y1);
     float part = 0.0f;
 #pragma unroll
     for (int v = 0; v < 4; v++)
-      part += w3[0][v] * fmaxf(y0[v] + b2[0][v], 0.0f) + w3[1][v] * fmaxf(y1[v] + b2[1][v], 0.0f);
+      part += w3[0][v] * relu(y0[v]) + w3[1][v] * relu(y1[v]);
     part += __shfl_xor(part, 16);
     part += __shfl_xor(part, 32);
     if (lq == ft) mine = part + b3;
