@@ -42,8 +42,10 @@ DT_BYTES_PER_NODE = (6 + 9) * 8
 
 def kernel_work(name, N):
     """(kind, units of work per launch) for the per-kernel roofline."""
-    faces = N * N * (N + 1)
-    if name in ("xfaces", "march_eta", "march_zeta"):
+    faces = N * N * (N + 1)  # faces of one axis (periodic: N+1 per line)
+    if name == "march_eta":  # xi- and eta-faces (xi fused into the eta march)
+        return "flop", 2 * faces * FLOPS_PER_FACE
+    if name in ("xfaces", "march_zeta"):
         return "flop", faces * FLOPS_PER_FACE
     if name == "visc_flux":
         return "byte", (N + 4) ** 3 * VISC_FLUX_BYTES_PER_NODE
@@ -55,17 +57,32 @@ def kernel_work(name, N):
     return "byte", ((N + 8) ** 3 - N ** 3) * 11 * 16
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_tgv256.json")
+
+
+def measured_traffic(name, N):
+    """HBM bytes per launch of kernel `name` from the committed rocprofv3
+    FETCH_SIZE + WRITE_SIZE summary of this workload (None if not profiled)."""
+    if N != 256 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    with open(TRAFFIC_FILE) as f:
+        t = json.load(f).get("bytes_per_launch", {})
+    return t.get(name)
+
+
 def roofline_of(name, ms, n, N):
     kind, work = kernel_work(name, N)
     avg_s = ms / n / 1e3
     if kind == "flop":
         ach = work / avg_s / 1e12
         return {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP64_TFLOPS, "traffic": None, "kernel": name, "avg_ms": ms / n,
+                "frac": ach / PEAK_FP64_TFLOPS, "traffic": measured_traffic(name, N), "kernel": name,
+                "avg_ms": ms / n,
                 "note": "FP64 VALU-bound face-flux sweep; peak = MI355X FP64 (vector = matrix)"}
     ach = work / avg_s / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": ach / PEAK_HBM_GBS, "traffic": None, "kernel": name, "avg_ms": ms / n}
+            "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(name, N), "kernel": name,
+            "avg_ms": ms / n}
 
 
 def log(*a):
