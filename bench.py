@@ -194,8 +194,8 @@ def main():
                     help="steps of the hybrid's error check against FP64 (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; multi-GPU halo exchange is a design "
-              "(DESIGN.md), not a measured path. Use --gpus 1.", file=sys.stderr)
+        print("bench.py: runs on one GPU; the multi-GPU halo exchange is designed, "
+              "not a measured path. Use --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     from cbns3d_cuda_amd import cases
