@@ -105,13 +105,15 @@ __device__ inline bool box_node3(const Box& b, int& i, int& j, int& k) {
 constexpr int N_XCD = CBNS_N_XCD;
 struct TileMarch {
   Box box;
-  int ntx, nty, nkc, chunk;
+  int ntx, nty, nkc, chunk, rows;
 };
-inline TileMarch tile_march(const Box& b, int chunk) {
+// rows: workgroup rows along j (the workgroup is 64 x rows threads)
+inline TileMarch tile_march(const Box& b, int chunk, int rows = BOX_Y) {
   TileMarch t;
   t.box = b;
+  t.rows = rows;
   t.ntx = (b.n[0] + BOX_X - 1) / BOX_X;
-  t.nty = (b.n[1] + BOX_Y - 1) / BOX_Y;
+  t.nty = (b.n[1] + rows - 1) / rows;
   t.nkc = (b.n[2] + chunk - 1) / chunk;
   t.chunk = chunk;
   return t;
@@ -127,7 +129,7 @@ __device__ inline bool tile_march_node(const TileMarch& t, int& i, int& j, int& 
   if (lb >= nb) return false;
   const unsigned tx = lb % t.ntx, r = lb / t.ntx;
   const unsigned ty = r % t.nty, kc = r / t.nty;
-  const int ti = tx * BOX_X + threadIdx.x, tj = ty * BOX_Y + threadIdx.y;
+  const int ti = tx * BOX_X + threadIdx.x, tj = ty * t.rows + threadIdx.y;
   if (ti >= t.box.n[0] || tj >= t.box.n[1]) return false;
   i = t.box.lo[0] + ti;
   j = t.box.lo[1] + tj;

@@ -26,6 +26,10 @@ namespace {
 #ifndef VISC_CHUNK
 #define VISC_CHUNK 8  // nodes along k per thread
 #endif
+#ifndef VISC_ROWS
+#define VISC_ROWS 4  // workgroup rows along j (64 x VISC_ROWS threads)
+#endif
+constexpr int VROWS = VISC_ROWS;
 
 // 4th-order derivative along axis a at node m of the viscous range:
 // one-sided at the two outermost layers of a reference axis
@@ -128,7 +132,7 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_visc_fluxes(Consts c, TileMarch tm,
+__global__ void __launch_bounds__(64 * VROWS) k_visc_fluxes(Consts c, TileMarch tm,
                                                      const double* __restrict__ prim,
                                                      const double* __restrict__ met,
                                                      double* __restrict__ mu,
@@ -181,7 +185,7 @@ __device__ __forceinline__ void visc_div_node(const Consts& c, int i, int j, int
 }
 
 template <class T>
-__global__ void __launch_bounds__(256) k_visc_div(Consts c, TileMarch tm,
+__global__ void __launch_bounds__(64 * VROWS) k_visc_div(Consts c, TileMarch tm,
                                                   const T* __restrict__ vf,
                                                   double* __restrict__ out, int mode) {
   int i, j, k0, k1;
@@ -221,8 +225,8 @@ int viscous(cbns_block* b, T* vf, double* mu, double* out, int mode) {
   }
   {
     ProfScope ps(b, CBNS_K_VISC_FLUX);
-    const TileMarch tm = tile_march(vb, VISC_CHUNK);
-    k_visc_fluxes<T><<<tile_march_blocks(tm), box_block(), 0, b->stream>>>(c, tm, b->prim,
+    const TileMarch tm = tile_march(vb, VISC_CHUNK, VROWS);
+    k_visc_fluxes<T><<<tile_march_blocks(tm), dim3(BOX_X, VROWS), 0, b->stream>>>(c, tm, b->prim,
                                                                           b->met, mu, vf);
     CBNS_HIP(b, hipGetLastError());
   }
@@ -232,8 +236,8 @@ int viscous(cbns_block* b, T* vf, double* mu, double* out, int mode) {
     ub.n[a] = upd_hi(c, a) - upd_lo(c, a) + 1;
   }
   ProfScope ps(b, CBNS_K_VISC_DIV);
-  const TileMarch tm = tile_march(ub, VISC_CHUNK);
-  k_visc_div<T><<<tile_march_blocks(tm), box_block(), 0, b->stream>>>(c, tm, vf, out, mode);
+  const TileMarch tm = tile_march(ub, VISC_CHUNK, VROWS);
+  k_visc_div<T><<<tile_march_blocks(tm), dim3(BOX_X, VROWS), 0, b->stream>>>(c, tm, vf, out, mode);
   CBNS_HIP(b, hipGetLastError());
   return CBNS_OK;
 }
