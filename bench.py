@@ -89,7 +89,7 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(N=128, steps=4):
+def cpu_baseline(N=256, steps=3):
     """The oracle on the host cores (bounded sample: TGV N^3, `steps` steps)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle import pyoracle as po
@@ -184,8 +184,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-size", type=int, default=128)
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-size", type=int, default=256)
+    ap.add_argument("--cpu-steps", type=int, default=3)
     ap.add_argument("--no-forms", action="store_true",
                     help="measure only the FP64 plain form")
     ap.add_argument("--indicator", default=os.path.join(ROOT, "cbns3d_cuda_amd",
