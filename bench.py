@@ -1,10 +1,14 @@
 """Benchmark: Mcell-RK3-steps/s of the explicit RK3 step on one MI355X.
 
-Workload: viscous Taylor-Green vortex, Re=1600, M=0.1, Pr=0.72, Sutherland
-viscosity, on [0,2 pi]^3 periodic, 256^3 nodes on a 5% sinusoidally perturbed
-curvilinear mesh (seeded phases).  One step = global CFL dt reduction + three
-SSP-RK3 stages (face fluxes, viscous terms, update, ghost fill), inputs
-resident in HBM.  Prints one JSON line.
+Default workload (the metric's): viscous Taylor-Green vortex, Re=1600,
+M=0.1, Pr=0.72, Sutherland viscosity, on [0,2 pi]^3 periodic, 256^3 nodes on
+a 5% sinusoidally perturbed curvilinear mesh (seeded phases).  --workload
+shock_vortex (Mach-2 shock / vortex, 256x128x128, inviscid) and hit
+(isotropic turbulence 512^3, viscous) measure the learned and combined forms'
+workloads.  One step = global CFL dt reduction + three SSP-RK3 stages (face
+fluxes, viscous terms, update, ghost fill), inputs resident in HBM.  Prints
+one JSON line: `value` is the plain FP64 form; `forms` adds the FP32 form and
+the FP32 + learned-hybrid form with the hybrid's error against FP64.
 
 The roofline object reports the kernel with the largest device time in the
 timed region (HIP events on the block's stream); cpu_baseline times the CPU
@@ -40,49 +44,69 @@ VISC_DIV_BYTES_PER_NODE = (12 + 2 * 4) * 8
 DT_BYTES_PER_NODE = (6 + 9) * 8
 
 
-def kernel_work(name, N):
+def face_counts(p):
+    """Faces per axis the fused stage computes, and updated nodes per axis."""
+    n = (p.IM, p.JM, p.KM)
+    per = [p.bc[a] == 1 for a in range(3)]
+    upd = [n[a] if per[a] else n[a] - 2 for a in range(3)]
+    faces = []
+    for a in range(3):
+        f = n[a] - 1 + (2 if per[a] else 0)
+        for b in range(3):
+            if b != a:
+                f *= upd[b]
+        faces.append(f)
+    return faces, upd
+
+
+def kernel_work(name, p):
     """(kind, units of work per launch) for the per-kernel roofline."""
-    faces = N * N * (N + 1)  # faces of one axis (periodic: N+1 per line)
+    faces, upd = face_counts(p)
+    nodes = upd[0] * upd[1] * upd[2]
     if name == "march_eta":  # xi- and eta-faces (xi fused into the eta march)
-        return "flop", 2 * faces * FLOPS_PER_FACE
-    if name in ("xfaces", "march_zeta"):
-        return "flop", faces * FLOPS_PER_FACE
+        return "flop", (faces[0] + faces[1]) * FLOPS_PER_FACE
+    if name == "xfaces":
+        return "flop", faces[0] * FLOPS_PER_FACE
+    if name == "march_zeta":
+        return "flop", faces[2] * FLOPS_PER_FACE
     if name == "visc_flux":
-        return "byte", (N + 4) ** 3 * VISC_FLUX_BYTES_PER_NODE
+        return "byte", (p.IM + 4) * (p.JM + 4) * (p.KM + 4) * VISC_FLUX_BYTES_PER_NODE
     if name == "visc_div":
-        return "byte", N ** 3 * VISC_DIV_BYTES_PER_NODE
+        return "byte", nodes * VISC_DIV_BYTES_PER_NODE
     if name == "dt":
-        return "byte", N ** 3 * DT_BYTES_PER_NODE
+        return "byte", p.IM * p.JM * p.KM * DT_BYTES_PER_NODE
+    if name == "indicator":  # per axis launch: 6-node stencil + 2 node states read once, 1 B out
+        return "byte", sum(faces) // 3 * (5 + 8) * 8
     # ghost shell of 4 layers: Q (5) + primitives (6) read and written
-    return "byte", ((N + 8) ** 3 - N ** 3) * 11 * 16
+    return "byte", ((p.IM + 8) * (p.JM + 8) * (p.KM + 8) - p.IM * p.JM * p.KM) * 11 * 16
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_tgv256.json")
 
 
-def measured_traffic(name, N):
+def measured_traffic(name, p, workload):
     """HBM bytes per launch of kernel `name` from the committed rocprofv3
     FETCH_SIZE + WRITE_SIZE summary of this workload (None if not profiled)."""
-    if N != 256 or not os.path.exists(TRAFFIC_FILE):
+    if workload != "tgv" or p.IM != 256 or not os.path.exists(TRAFFIC_FILE):
         return None
     with open(TRAFFIC_FILE) as f:
         t = json.load(f).get("bytes_per_launch", {})
     return t.get(name)
 
 
-def roofline_of(name, ms, n, N):
-    kind, work = kernel_work(name, N)
+def roofline_of(name, ms, n, p, workload):
+    kind, work = kernel_work(name, p)
     avg_s = ms / n / 1e3
     if kind == "flop":
         ach = work / avg_s / 1e12
         return {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "frac": ach / PEAK_FP64_TFLOPS, "traffic": measured_traffic(name, N), "kernel": name,
-                "avg_ms": ms / n,
+                "frac": ach / PEAK_FP64_TFLOPS, "traffic": measured_traffic(name, p, workload),
+                "kernel": name, "avg_ms": ms / n,
                 "note": "FP64 VALU-bound face-flux sweep; peak = MI355X FP64 (vector = matrix)"}
     ach = work / avg_s / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-            "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(name, N), "kernel": name,
-            "avg_ms": ms / n}
+            "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(name, p, workload),
+            "kernel": name, "avg_ms": ms / n}
 
 
 def log(*a):
@@ -145,17 +169,18 @@ def run_form(case, form, params, thresh, steps, warmup, profile=True):
     return el, kt, finite
 
 
-def other_forms(case, args):
+def other_forms(case, check_case, args):
     """FP32 and FP32 + learned hybrid: rates over the same timed steps, and
     the hybrid's rho/E error against the FP64 form after `check_steps` steps
-    from the same initial state."""
+    from the same initial state of `check_case`."""
     from cbns3d_cuda_amd import learned
     from cbns3d_cuda_amd.block import FORM_FP32, FORM_HYBRID
 
-    N = case.params.IM
+    p = case.params
+    cells = p.IM * p.JM * p.KM
     out = {}
     el, kt, finite = run_form(case, FORM_FP32, None, 0.0, args.steps, args.warmup, profile=False)
-    out["fp32"] = {"value": N ** 3 * args.steps / el / 1e6, "ms_per_step": 1e3 * el / args.steps,
+    out["fp32"] = {"value": cells * args.steps / el / 1e6, "ms_per_step": 1e3 * el / args.steps,
                    "finite": finite}
     if not os.path.exists(args.indicator):
         out["fp32_hybrid"] = {"skipped": "no indicator weights at %s" % args.indicator}
@@ -163,13 +188,14 @@ def other_forms(case, args):
     P, thr, meta = learned.load(args.indicator)
     form = FORM_FP32 | FORM_HYBRID
     el, kt, finite = run_form(case, form, P, thr, args.steps, args.warmup)
-    h = {"value": N ** 3 * args.steps / el / 1e6, "ms_per_step": 1e3 * el / args.steps,
+    h = {"value": cells * args.steps / el / 1e6, "ms_per_step": 1e3 * el / args.steps,
          "finite": finite, "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in kt.items() if v[1]},
          "indicator": os.path.basename(args.indicator), "threshold": thr}
     if args.check_steps > 0:
         log("bench: hybrid error check, %d steps" % args.check_steps)
-        err = learned.compare(case, args.check_steps, form, P, thr, log=log)
-        h.update({"check_steps": args.check_steps, "L2_rho": err["L2_rho"], "Linf_rho": err["Linf_rho"],
+        err = learned.compare(check_case, args.check_steps, form, P, thr, log=log)
+        h.update({"check_steps": args.check_steps, "check_case": check_case.name,
+                  "L2_rho": err["L2_rho"], "Linf_rho": err["Linf_rho"],
                   "L2_E": err["L2_E"], "Linf_E": err["Linf_E"],
                   "troubled_fraction_mean": err.get("troubled_fraction_mean"),
                   "meets_L2_rho_1e-6": err["L2_rho"] <= 1e-6})
@@ -182,7 +208,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--workload", choices=("tgv", "shock_vortex", "hit"), default="tgv",
+                    help="tgv: viscous Taylor-Green 256^3 (the metric's workload); "
+                         "shock_vortex: Mach-2 shock/vortex 256x128x128 (learned form); "
+                         "hit: isotropic turbulence 512^3 (combined form)")
+    ap.add_argument("--size", type=int, default=0, help="N (default: the workload's own)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=256)
     ap.add_argument("--cpu-steps", type=int, default=3)
@@ -200,11 +230,27 @@ def main():
 
     from cbns3d_cuda_amd import cases
 
-    N = args.size
-    log("bench: building taylor_green %d^3" % N)
-    case = cases.taylor_green(N)
+    wl = args.workload
+    N = args.size or {"tgv": 256, "shock_vortex": 256, "hit": 512}[wl]
+    if wl == "tgv":
+        case = cases.taylor_green(N)
+        check = case
+        desc = ("viscous Taylor-Green vortex %d^3, Re=1600, M=0.1, curvilinear periodic" % N)
+    elif wl == "shock_vortex":
+        case = cases.shock_vortex(N, N // 2, N // 2, seed=0)
+        check = case
+        desc = ("Mach-2 shock / compressible vortex %dx%dx%d, inviscid, clustered curvilinear "
+                "mesh" % (N, N // 2, N // 2))
+    else:
+        case = cases.isotropic_turbulence(N, seed=0)
+        check = cases.isotropic_turbulence(128, seed=0)
+        desc = ("decaying isotropic turbulence %d^3, Re_lambda=100, Mt=0.6, periodic; hybrid error "
+                "on a 128^3 replay" % N)
+    p = case.params
+    cells = p.IM * p.JM * p.KM
+    log("bench: building %s" % case.name)
     el, kt, finite = run_form(case, 0, None, 0.0, args.steps, args.warmup)
-    value = N ** 3 * args.steps / el / 1e6
+    value = cells * args.steps / el / 1e6
     log("bench: %.3f ms/step, %.1f Mcell-RK3-steps/s, finite=%s" % (1e3 * el / args.steps, value, finite))
     for k, (ms, n) in kt.items():
         if n:
@@ -212,20 +258,19 @@ def main():
 
     # dominant kernel and its algorithmic work per launch; every kernel's
     # roofline is listed beside it
-    nodes = N ** 3
     dom = max(kt, key=lambda k: kt[k][0])
-    roof = roofline_of(dom, kt[dom][0], kt[dom][1], N)
-    per_kernel = {k: roofline_of(k, ms, n, N) for k, (ms, n) in kt.items() if n}
+    roof = roofline_of(dom, kt[dom][0], kt[dom][1], p, wl)
+    per_kernel = {k: roofline_of(k, ms, n, p, wl) for k, (ms, n) in kt.items() if n}
 
     forms = None
     if not args.no_forms:
-        forms = other_forms(case, args)
+        forms = other_forms(case, check, args)
         for k, v in forms.items():
             log("  form %s: %s" % (k, json.dumps(v)))
-    del case
+    del case, check
 
     cpu = None
-    if not args.no_cpu_baseline:
+    if not args.no_cpu_baseline and wl == "tgv":
         log("bench: cpu baseline (oracle)")
         cpu = cpu_baseline(args.cpu_size, args.cpu_steps)
 
@@ -241,9 +286,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (analytic Taylor-Green initial state, seeded mesh phases)",
-        "config": {"workload": "viscous Taylor-Green vortex %d^3, Re=1600, M=0.1, curvilinear "
-                               "periodic, plain FP64 form" % N, "nodes": nodes,
+        "data": "synthetic (analytic / seeded initial state and mesh, no dataset)",
+        "config": {"workload": desc + "; plain FP64 form", "nodes": cells,
                    "form": "fp64-plain", "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,

@@ -22,6 +22,11 @@ initial state at the nodes.  Nothing here runs per step.
                      the shock plane and gently curved; xi takes the
                      reference's far-field rule, eta and zeta are periodic;
                      vortex centre and sense of rotation from a seeded PCG64
+  isotropic_turbulence  decaying compressible isotropic turbulence on
+                     [0,2 pi]^3, periodic, uniform mesh: a random solenoidal
+                     Fourier velocity field with E(k) ~ k^4 exp(-2 (k/k0)^2),
+                     k0 = 4, seeded phases, scaled to turbulent Mach 0.6 and
+                     Re_lambda = 100; uniform rho and p; viscous
 """
 from dataclasses import dataclass
 
@@ -184,6 +189,71 @@ def shock_vortex(IM=256, JM=128, KM=128, *, seed=0, CFL=0.5, Mach=2.0, gamma=1.4
     period = np.diag([0.0, Ly, Lz])
     return Case("shock_vortex_%dx%dx%d_s%d" % (IM, JM, KM, seed), params, x, y, z, period,
                 np.stack([rho, u, v, w, p]))
+
+
+def solenoidal_field(N, seed, k0=4.0):
+    """Real velocity field (3, N, N, N) on the uniform grid x = 2 pi i / N
+    (axis order k, j, i), divergence-free mode by mode, with shell energy
+    E(k) ~ k^4 exp(-2 (k/k0)^2) and complex-Gaussian mode amplitudes from a
+    seeded PCG64; normalised to <|u|^2> = 1."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    kf = np.fft.fftfreq(N, 1.0 / N)
+    kr = np.fft.rfftfreq(N, 1.0 / N)
+    kz = kf[:, None, None]
+    ky = kf[None, :, None]
+    kx = kr[None, None, :]
+    k2 = kx * kx + ky * ky + kz * kz
+    kk = np.sqrt(k2)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        amp = np.where(kk > 0, np.sqrt(kk ** 4 * np.exp(-2.0 * (kk / k0) ** 2) / (4.0 * np.pi * k2)),
+                       0.0)
+    shape = amp.shape
+    uh = [amp * (rng.standard_normal(shape) + 1j * rng.standard_normal(shape)) for _ in range(3)]
+    # solenoidal projection: u_hat -= k (k . u_hat) / |k|^2  (components z, y, x -> w, v, u)
+    kvec = (kx, ky, kz)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        dot = (kx * uh[0] + ky * uh[1] + kz * uh[2]) / np.where(k2 > 0, k2, 1.0)
+    for c in range(3):
+        uh[c] = uh[c] - kvec[c] * dot
+    u = np.stack([np.fft.irfftn(uh[c], s=(N, N, N), axes=(0, 1, 2)) for c in range(3)])  # u, v, w
+    del uh
+    u /= np.sqrt(np.mean(np.sum(u * u, axis=0)))
+    return u
+
+
+def taylor_microscale(u):
+    """Longitudinal Taylor microscale of a periodic field on [0, 2 pi]^3:
+    lambda = u' / sqrt(<(du_a/dx_a)^2>), u' = sqrt(<|u|^2>/3)."""
+    N = u.shape[-1]
+    kf = np.fft.fftfreq(N, 1.0 / N)
+    kr = np.fft.rfftfreq(N, 1.0 / N)
+    ks = (kr[None, None, :], kf[None, :, None], kf[:, None, None])  # x, y, z
+    g2 = []
+    for a in range(3):
+        d = np.fft.irfftn(1j * ks[a] * np.fft.rfftn(u[a]), s=u.shape[1:], axes=(0, 1, 2))
+        g2.append(np.mean(d * d))
+    uprime = np.sqrt(np.mean(np.sum(u * u, axis=0)) / 3.0)
+    return uprime, uprime / np.sqrt(np.mean(g2))
+
+
+def isotropic_turbulence(N=512, *, seed=0, Mt=0.6, Re_lambda=100.0, k0=4.0, CFL=0.5,
+                         gamma=1.4, Pr=0.72, T_inf=273.15):
+    """Velocities scaled to rms magnitude 1 and Mach = Mt, so the turbulent
+    Mach number sqrt(<|u|^2>)/c is Mt; rho = 1, T = 1 (mu = 1); Re from
+    Re_lambda = u' lambda / nu with nu = 1/Re."""
+    L = 2.0 * np.pi
+    u = solenoidal_field(N, seed, k0)
+    uprime, lam = taylor_microscale(u)
+    Re = Re_lambda / (uprime * lam)
+    s = _node_coords(N, L)
+    zeta, eta, xi = np.meshgrid(s, s, s, indexing="ij")
+    rho = np.ones_like(xi)
+    p = np.full_like(xi, 1.0 / (gamma * Mt * Mt))
+    params = make_params(N, N, N, viscous=1, bc=(BC_PERIODIC,) * 3, CFL=CFL, gamma=gamma,
+                         Mach=Mt, Re=Re, Pr=Pr, T_inf=T_inf)
+    period = np.eye(3) * L
+    return Case("isotropic_turbulence_%d_s%d" % (N, seed), params, xi, eta, zeta, period,
+                np.concatenate([rho[None], u, p[None]]))
 
 
 def pad_modes(params):

@@ -109,13 +109,32 @@ void internal_to_ref(const Consts& c, const std::vector<double>& src, double* ds
       }
 }
 
+// lazily allocated scratch (zero-filled like every array): the face fluxes
+// and dfv exist only for the reference-decomposed calls, the viscous-flux
+// scratch for any viscous stage
+int lazy_alloc(cbns_block* b, double** p, long n) {
+  if (*p) return CBNS_OK;
+  int rc = dmalloc(b, p, n);
+  if (rc) return rc;
+  if (launch_fill(b->stream, *p, n, 0.0)) return fail(b, CBNS_ERR_HIP, "fill failed");
+  return CBNS_OK;
+}
+
+int ensure_stage_scratch(cbns_block* b) {
+  const long S = b->c.S;
+  int rc;
+  if (fused_stage_uses_face_flux() && (rc = lazy_alloc(b, &b->flux, 15 * S))) return rc;
+  if (b->c.viscous && (rc = lazy_alloc(b, &b->vtmp, 12 * S))) return rc;
+  return CBNS_OK;
+}
+
 int ensure_unfused(cbns_block* b) {
   const long S = b->c.S;
   int rc;
-  if (!b->flux && (rc = dmalloc(b, &b->flux, 15 * S))) return rc;
+  if ((rc = lazy_alloc(b, &b->flux, 15 * S))) return rc;
   if (b->c.viscous) {
-    if (!b->dfv && (rc = dmalloc(b, &b->dfv, 4 * S))) return rc;
-    if (!b->vtmp && (rc = dmalloc(b, &b->vtmp, 12 * S))) return rc;
+    if ((rc = lazy_alloc(b, &b->dfv, 4 * S))) return rc;
+    if ((rc = lazy_alloc(b, &b->vtmp, 12 * S))) return rc;
   }
   return CBNS_OK;
 }
@@ -160,7 +179,7 @@ int cbns_create(const cbns_params* p, cbns_block** out) {
   if ((rc = dmalloc(b, &b->met, 10 * S)) || (rc = dmalloc(b, &b->Q, 5 * S)) ||
       (rc = dmalloc(b, &b->qp, 5 * S)) || (rc = dmalloc(b, &b->qspare, 5 * S)) ||
       (rc = dmalloc(b, &b->prim, 6 * S)) || (rc = dmalloc(b, &b->mu, S)) ||
-      (rc = dmalloc(b, &b->rhs, 5 * S)) || (rc = dmalloc(b, &b->flux, 15 * S))) {
+      (rc = dmalloc(b, &b->rhs, 5 * S))) {
     cbns_destroy(b);
     return rc;
   }
@@ -175,9 +194,9 @@ int cbns_create(const cbns_params* p, cbns_block** out) {
     return CBNS_ERR_HIP;
   }
   // defined contents everywhere (ghost corners a boundary rule never writes)
-  double* zs[] = {b->met, b->Q, b->qp, b->qspare, b->prim, b->mu, b->rhs, b->flux};
-  const long zn[] = {10 * S, 5 * S, 5 * S, 5 * S, 6 * S, S, 5 * S, 15 * S};
-  for (int q = 0; q < 8; q++)
+  double* zs[] = {b->met, b->Q, b->qp, b->qspare, b->prim, b->mu, b->rhs};
+  const long zn[] = {10 * S, 5 * S, 5 * S, 5 * S, 6 * S, S, 5 * S};
+  for (int q = 0; q < 7; q++)
     if (launch_fill(b->stream, zs[q], zn[q], 0.0)) {
       cbns_destroy(b);
       return CBNS_ERR_HIP;
@@ -408,7 +427,7 @@ int cbns_rk3_stage(cbns_block* b, double dt, int32_t stage) {
   int rc = check_block(b);
   if (rc) return rc;
   if (stage < 1 || stage > 3) return fail(b, CBNS_ERR_ARG, "stage must be 1, 2 or 3");
-  if (b->c.viscous && (rc = ensure_unfused(b))) return rc;
+  if ((rc = ensure_stage_scratch(b))) return rc;
   // stage 1: Q -> qp; stage 2: qp -> qspare, then qspare becomes qp;
   // stage 3: qp -> Q.  Boundary nodes no rule writes hold identical values
   // in all three buffers (set/calc_conservative copy Q into each).
@@ -507,6 +526,7 @@ int cbns_flag_count(cbns_block* b, int64_t counts[3]) {
 int cbns_get_field(cbns_block* b, int32_t f, double* host) {
   if (!b || !host) return CBNS_ERR_ARG;
   CBNS_HIP(b, hipSetDevice(b->device));
+  if (f >= CBNS_F_FLUX && ensure_unfused(b)) return CBNS_ERR_HIP;
   double* d = field_ptr(b, f);
   if (!d) return fail(b, CBNS_ERR_ARG, "field not available");
   CBNS_HIP(b, hipMemcpyAsync(host, d, sizeof(double) * b->c.S, hipMemcpyDeviceToHost, b->stream));

@@ -164,7 +164,7 @@ struct cbns_block {
   double* prim = nullptr;    // 6*S
   double* mu = nullptr;      // S
   double* rhs = nullptr;     // 5*S, fused-stage accumulator
-  double* flux = nullptr;    // 15*S (lazy, unfused path)
+  double* flux = nullptr;    // 15*S (lazy: reference-decomposed calls)
   double* dfv = nullptr;     // 4*S  (lazy)
   double* vtmp = nullptr;    // 12*S (lazy): viscous fluxes Ev,Fv,Gv
   double* red = nullptr;     // reduction scratch
@@ -223,6 +223,7 @@ int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs);
 int launch_update_rk3(cbns_block* b, double dt, int stage);
 int launch_calc_dt(cbns_block* b, double* dt_host);
 int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout);
+bool fused_stage_uses_face_flux();
 int launch_fill(hipStream_t s, double* p, long n, double v);
 long face_count(const cbns_block* b, int axis);
 int launch_face_samples(cbns_block* b, int axis, const double* Q, float* feat, float* label);

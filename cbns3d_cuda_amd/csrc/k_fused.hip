@@ -286,6 +286,8 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
   return CBNS_OK;
 }
 
+bool fused_stage_uses_face_flux() { return !FUSE_XI; }
+
 int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout) {
   switch (b->form) {
     case CBNS_FORM_FP32: return fused_stage<float, false>(b, dt, stage, qin, qout);

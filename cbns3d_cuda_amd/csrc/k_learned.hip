@@ -98,7 +98,10 @@ __global__ void __launch_bounds__(256) k_indicator(Consts c, Box box, const doub
   const bool flag = valid && s > thresh;
   if (valid) flags[a] = flag ? 1 : 0;
   if (count) {
-    const unsigned long long bal = __ballot(flag);
+    // count faces of the flux kernels' range, not the one-face halo
+    const int pa = AX == 0 ? i : (AX == 1 ? j : k);
+    const bool inner = pa > box.lo[AX] && pa < box.lo[AX] + box.n[AX] - 1;
+    const unsigned long long bal = __ballot(flag && inner);
     if ((threadIdx.x & 63) == 0 && bal) atomicAdd(count, (unsigned)__popcll(bal));
   }
 }

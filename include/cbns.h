@@ -193,7 +193,8 @@ int cbns_face_samples(cbns_block* blk, int32_t axis, int32_t buf, float* d_feat,
  * (cbns_flag_count). */
 int cbns_set_indicator(cbns_block* blk, const float* params, float threshold, int32_t options);
 /* troubled faces (before dilation) per axis found by the last indicator pass
- * (needs options bit 1) */
+ * among the faces the flux kernels compute (cbns_face_count); needs options
+ * bit 1 */
 int cbns_flag_count(cbns_block* blk, int64_t counts[3]);
 /* the troubled-face bytes of the last indicator pass: 3 ghosted fields of
  * uint8 (axis-major; face m of an axis, between nodes m and m+1, at node m),
