@@ -156,6 +156,17 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
       for (int n = 0; n < 5; n++) q[s][n] = q[s + 1][n];
     o = base + m * st;
     load_q(Q, S, o + 3 * st, q[5]);
+    // the zeta sweep's update operands are fetched before the face work so
+    // their latency overlaps it
+    double rin[5], qnv[5], jo = 0.0;
+    if (AX == 2) {
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+        rin[n] = sa.rhs[n * S + o];
+        qnv[n] = sa.stage > 1 ? sa.Qn[n * S + o] : 0.0;
+      }
+      jo = J[o];
+    }
     ax = bx; ay = by; az = bz;
     jb = J[o + st];
     bx = nX[o + st] * jb; by = nY[o + st] * jb; bz = nZ[o + st] * jb;
@@ -199,18 +210,18 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
         sa.rhs[n * S + o] = (fxn[o] - fxn[o - 1]) + (f[n] - fprev[n]);
       }
     } else {
-      const double inv_j = 1.0 / J[o];
+      const double inv_j = 1.0 / jo;
       double qn_out[5];
 #pragma unroll
       for (int n = 0; n < 5; n++) {
-        double df = sa.rhs[n * S + o] + (f[n] - fprev[n]);
+        double df = rin[n] + (f[n] - fprev[n]);
         if (n > 0 && sa.dfv) df -= sa.dfv[(n - 1) * S + o];
         df *= inv_j;
         const double qin = q[2][n];
         double out;
         if (sa.stage == 1) out = qin - sa.dt * df;
-        else if (sa.stage == 2) out = 0.75 * sa.Qn[n * S + o] + 0.25 * (qin - sa.dt * df);
-        else out = (sa.Qn[n * S + o] + 2.0 * (qin - sa.dt * df)) * (1.0 / 3.0);
+        else if (sa.stage == 2) out = 0.75 * qnv[n] + 0.25 * (qin - sa.dt * df);
+        else out = (qnv[n] + 2.0 * (qin - sa.dt * df)) * (1.0 / 3.0);
         sa.Qout[n * S + o] = out;
         qn_out[n] = out;
       }
