@@ -256,6 +256,43 @@ def isotropic_turbulence(N=512, *, seed=0, Mt=0.6, Re_lambda=100.0, k0=4.0, CFL=
                 np.concatenate([rho[None], u, p[None]]))
 
 
+def smooth_box(IM, JM, KM, *, viscous=1, bc=(BC_PERIODIC,) * 3, seed=11, Mach=0.3, CFL=0.5,
+               visc_centered=0):
+    """Any size and boundary mix: [0, 2 pi)^3 in computational coordinates,
+    curved by x = xi + 0.2 sin(eta + a) sin(zeta + b) and its cyclic
+    permutations (periodic in every index), with a smooth seeded state of a
+    few Fourier modes around a freestream (rho, p > 0 everywhere)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    L = 2.0 * np.pi
+    zeta, eta, xi = np.meshgrid(_node_coords(KM, L), _node_coords(JM, L), _node_coords(IM, L),
+                                indexing="ij")
+    ph = rng.uniform(0.0, 2.0 * np.pi, size=6)
+    x = xi + 0.2 * np.sin(eta + ph[0]) * np.sin(zeta + ph[1])
+    y = eta + 0.2 * np.sin(zeta + ph[2]) * np.sin(xi + ph[3])
+    z = zeta + 0.2 * np.sin(xi + ph[4]) * np.sin(eta + ph[5])
+    gamma = 1.4
+
+    def modes(amp):
+        a = rng.uniform(-1.0, 1.0, size=(3, 3))
+        return amp * (np.sin(xi + a[0, 0]) * np.cos(eta + a[0, 1]) +
+                      np.cos(eta + a[1, 0]) * np.sin(zeta + a[1, 1]) +
+                      np.sin(zeta + a[2, 0] + xi) * a[2, 2])
+
+    rho = 1.0 + modes(0.1)
+    u = 1.0 + modes(0.2)
+    v = 0.2 + modes(0.2)
+    w = -0.1 + modes(0.2)
+    p = 1.0 / (gamma * Mach ** 2) * (1.0 + modes(0.05))
+    n = (IM, JM, KM)
+    params = make_params(IM, JM, KM, viscous=viscous, visc_centered=visc_centered, bc=tuple(bc),
+                         CFL=CFL, gamma=gamma, Mach=Mach, Re=800.0, Pr=0.72, T_inf=273.15,
+                         i_begin=max(1, IM // 4), i_end=IM - max(1, IM // 4))
+    period = np.eye(3) * L
+    name = "smooth_box_%dx%dx%d_%s%s" % (IM, JM, KM, "".join("p" if b else "r" for b in bc),
+                                         "_visc" if viscous else "")
+    return Case(name, params, x, y, z, period, np.stack([rho, u, v, w, p]))
+
+
 def pad_modes(params):
     """np.pad mode per node axis (K, J, I): wrap on periodic axes, edge else."""
     return ["wrap" if params.bc[a] == BC_PERIODIC else "edge" for a in (2, 1, 0)]

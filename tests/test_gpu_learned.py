@@ -131,10 +131,12 @@ def _stage_pair(oracle, case, params, thresh, form):
     return o, b, dt
 
 
-@pytest.mark.parametrize("builder", ["vortex", "tgv", "channel"])
+@pytest.mark.parametrize("builder", ["vortex", "tgv", "channel", "box_rpp", "box_odd"])
 def test_hybrid_stage_matches_oracle(oracle, builder):
     case = {"vortex": lambda: cases.isentropic_vortex(16), "tgv": lambda: cases.taylor_green(16),
-            "channel": lambda: cases.c_grid_channel(viscous=1)}[builder]()
+            "channel": lambda: cases.c_grid_channel(viscous=1),
+            "box_rpp": lambda: cases.smooth_box(70, 9, 8, viscous=1, bc=(0, 1, 1)),
+            "box_odd": lambda: cases.smooth_box(65, 6, 5, viscous=0)}[builder]()
     print("case", case.name, flush=True)
     P = random_params(11)
     # threshold at the median logit of the oracle's network on this state's faces

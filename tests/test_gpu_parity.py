@@ -13,7 +13,8 @@ import numpy as np
 import pytest
 
 from cbns3d_cuda_amd import cases
-from cbns3d_cuda_amd.block import (BUF_Q, BUF_QP, F_DFV, F_FLUX, F_MET, F_PRIM, F_Q, F_QP, Block)
+from cbns3d_cuda_amd.block import (BC_PERIODIC, BC_REF, BUF_Q, BUF_QP, F_DFV, F_FLUX, F_MET, F_PRIM,
+                                   F_Q, F_QP, Block)
 from helpers import G, oracle_block, rel_err, update_mask
 
 pytestmark = pytest.mark.gpu
@@ -26,6 +27,13 @@ CASES = {
     "tgv_centered": lambda: cases.taylor_green(16, visc_centered=1),
     "cgrid_inviscid": lambda: cases.c_grid_channel(viscous=0),
     "cgrid_viscous": lambda: cases.c_grid_channel(viscous=1),
+    # edge sizes: the minimum, lines crossing the 63-node wavefront tiling,
+    # mixed boundary kinds
+    "box_5x6x7": lambda: cases.smooth_box(5, 6, 7, viscous=0),
+    "box_65x6x5_visc": lambda: cases.smooth_box(65, 6, 5, viscous=1),
+    "box_70x9x8_rpp_visc": lambda: cases.smooth_box(70, 9, 8, viscous=1,
+                                                    bc=(BC_REF, BC_PERIODIC, BC_PERIODIC)),
+    "box_7x9x6_rrr_visc": lambda: cases.smooth_box(7, 9, 6, viscous=1, bc=(BC_REF,) * 3),
 }
 
 
