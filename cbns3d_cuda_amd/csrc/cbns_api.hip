@@ -246,11 +246,12 @@ int cbns_set_mesh(cbns_block* b, const double* x, const double* y, const double*
   int rc;
   if ((rc = dmalloc(b, &d, 3 * NN))) return rc;
   const size_t nb = sizeof(double) * (size_t)NN;
-  CBNS_HIP(b, hipMemcpyAsync(d, x, nb, hipMemcpyHostToDevice, b->stream));
-  CBNS_HIP(b, hipMemcpyAsync(d + NN, y, nb, hipMemcpyHostToDevice, b->stream));
-  CBNS_HIP(b, hipMemcpyAsync(d + 2 * NN, z, nb, hipMemcpyHostToDevice, b->stream));
+  hipError_t e = hipMemcpyAsync(d, x, nb, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d + NN, y, nb, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(d + 2 * NN, z, nb, hipMemcpyHostToDevice, b->stream);
   int bad = 0;
-  rc = launch_metrics(b, d, d + NN, d + 2 * NN, period, &bad);
+  rc = e == hipSuccess ? launch_metrics(b, d, d + NN, d + 2 * NN, period, &bad)
+                       : hip_fail(b, e, "coordinate upload");
   (void)hipStreamSynchronize(b->stream);
   (void)hipFree(d);
   if (rc) return rc;
