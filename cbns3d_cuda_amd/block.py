@@ -301,10 +301,10 @@ class Block:
         """Features [n][32] and WENO-weight labels [n] of axis `axis`."""
         self._chk(lib().cbns_face_samples(self._h, axis, buf, d_feat, d_label))
 
-    def set_indicator(self, params, threshold=0.0, per_stage=False, count=False):
+    def set_indicator(self, params, threshold=0.0, per_stage=False, count=False, reuse=False):
         params = np.ascontiguousarray(params, dtype=np.float32)
         assert params.size == MLP_NPARAM
-        opts = (1 if per_stage else 0) | (2 if count else 0)
+        opts = (1 if per_stage else 0) | (2 if count else 0) | (4 if reuse else 0)
         self._chk(lib().cbns_set_indicator(self._h, _fptr(params), float(threshold), opts))
 
     def get_flags(self):

@@ -481,7 +481,7 @@ int cbns_face_samples(cbns_block* b, int32_t axis, int32_t buf, float* d_feat, f
 }
 
 int cbns_set_indicator(cbns_block* b, const float* params, float threshold, int32_t options) {
-  if (!b || !params || !std::isfinite(threshold) || options < 0 || options > 3) return CBNS_ERR_ARG;
+  if (!b || !params || !std::isfinite(threshold) || options < 0 || options > 7) return CBNS_ERR_ARG;
   CBNS_HIP(b, hipSetDevice(b->device));
   if (!b->flags) {
     CBNS_HIP(b, hipMalloc((void**)&b->flags, 3 * (size_t)b->c.S));
@@ -500,6 +500,8 @@ int cbns_set_indicator(cbns_block* b, const float* params, float threshold, int3
   b->ind_thresh = threshold;
   b->ind_per_stage = (options & 1) != 0;
   b->ind_count_on = (options & 2) != 0;
+  b->ind_every = (options & 4) ? std::max(1, (int)std::floor(1.0 / b->p.CFL)) : 1;
+  b->ind_age = 0;
   b->have_indicator = true;
   b->flags_valid = false;
   return CBNS_OK;

@@ -175,6 +175,8 @@ struct cbns_block {
   bool flags_valid = false;
   bool ind_count_on = false;
   bool ind_per_stage = false;
+  int ind_every = 1;             // steps between indicator passes
+  int ind_age = 0;               // steps since the last pass
   float ind_thresh = 0.0f;
   float* ind_P = nullptr;        // network parameters (fp32)
   void* ind_pack = nullptr;      // bf16 operand tiles (3 x 256 x 8 bytes)

@@ -248,9 +248,17 @@ template <class F, bool HYB>
 int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout) {
   const Consts& c = b->c;
   double* fx = b->flux;  // first 5 face arrays
-  if (HYB && (stage == 1 || b->ind_per_stage || !b->flags_valid)) {
-    int rc = launch_indicator(b, qin);
-    if (rc) return rc;
+  if (HYB) {
+    // once per step on the stage-1 input (or every ind_every steps, or at
+    // every stage)
+    const bool due = !b->flags_valid || b->ind_per_stage ||
+                     (stage == 1 && b->ind_age >= b->ind_every);
+    if (due) {
+      int rc = launch_indicator(b, qin);
+      if (rc) return rc;
+      b->ind_age = 0;
+    }
+    if (stage == 1) b->ind_age++;
   }
   const long S = c.S;
   const uint8_t* fl = HYB ? b->flags : nullptr;

@@ -165,7 +165,7 @@ def stats_at(s, target, thr):
             "flagged_fraction": float(pred.mean()), "troubled_fraction": float(pos.mean())}
 
 
-def compare(case, steps, form, params=None, thresh=0.0, per_stage=False, log=print):
+def compare(case, steps, form, params=None, thresh=0.0, per_stage=False, log=print, reuse=False):
     """Run `steps` RK3 steps with the plain FP64 form and with `form` from the
     same state; L2/Linf of rho and E (updated nodes) and troubled counts."""
     out = {}
@@ -174,7 +174,7 @@ def compare(case, steps, form, params=None, thresh=0.0, per_stage=False, log=pri
         b = start_block(case)
         try:
             if f & FORM_HYBRID:
-                b.set_indicator(params, thresh, per_stage=per_stage, count=True)
+                b.set_indicator(params, thresh, per_stage=per_stage, count=True, reuse=reuse)
             b.set_form(f)
             t = 0.0
             flagged = []

@@ -190,7 +190,9 @@ int cbns_face_samples(cbns_block* blk, int32_t axis, int32_t buf, float* d_feat,
  * axis run characteristic WENO5, the others the linear 5th-order upwind
  * reconstruction.  options bit 0: evaluate the indicator at every stage
  * (default: once per step, on the stage-1 input); bit 1: count troubled faces
- * (cbns_flag_count). */
+ * (cbns_flag_count); bit 2: keep the flags for floor(1/CFL) steps (a wave
+ * moves at most CFL cells per step along each axis, so within that many steps
+ * it stays inside the one-cell dilation). */
 int cbns_set_indicator(cbns_block* blk, const float* params, float threshold, int32_t options);
 /* troubled faces (before dilation) per axis found by the last indicator pass
  * among the faces the flux kernels compute (cbns_face_count); needs options

@@ -216,3 +216,45 @@ def test_fp32_hybrid_stage_close_to_oracle(oracle):
         assert err < 1e-7
     finally:
         b.free()
+
+
+def test_flag_reuse_over_steps(oracle):
+    """options bit 2: the flags of a step's indicator pass serve floor(1/CFL)
+    = 2 steps; the second step equals the oracle's hybrid step with them."""
+    case = cases.isentropic_vortex(16)  # CFL 0.5
+    print("case", case.name, flush=True)
+    P = random_params(13)
+    o = oracle_block(oracle, case)
+    o.bc_primitive()
+    b = Block(case.params)
+    try:
+        b.calc_metrics(case.x, case.y, case.z, case.period)
+        b.set_indicator(P, 0.0, count=True, reuse=True)
+        b.set_form(FORM_HYBRID)
+        b.set_profiling(True)
+        b.set_field(F_Q, o.Q)
+        b.set_field(F_QP, o.Qp)
+        b.set_field(F_PRIM, o.prim)
+        t = 0.0
+        t, dt1 = b.rk3_step(1e9, t)
+        assert b.kernel_times(reset=False)["indicator"][1] == 1
+        f1 = b.get_flags()
+        assert 0 < f1.sum()
+        # the oracle follows the GPU's first step with the same flags
+        dt = o.calc_dt()
+        assert dt == pytest.approx(dt1, rel=1e-12)
+        for s in (1, 2, 3):
+            o.rk3_stage_hybrid(dt, s, f1)
+        t, dt2 = b.rk3_step(1e9, t)
+        assert np.array_equal(b.get_flags(), f1)  # not re-evaluated
+        assert b.kernel_times(reset=False)["indicator"][1] == 1
+        dt = o.calc_dt()
+        for s in (1, 2, 3):
+            o.rk3_stage_hybrid(dt, s, f1)
+        err = rel_err(b.get_field(F_Q, 5), o.Q)
+        print("two steps with reused flags, relative error", err, flush=True)
+        assert err < 1e-11
+        t, _ = b.rk3_step(1e9, t)  # third step: a new pass
+        assert b.kernel_times(reset=False)["indicator"][1] == 2
+    finally:
+        b.free()
