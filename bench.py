@@ -218,10 +218,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", choices=("tgv", "shock_vortex", "hit"), default="tgv",
+    ap.add_argument("--workload", choices=("tgv", "shock_vortex", "hit", "slab"), default="tgv",
                     help="tgv: viscous Taylor-Green 256^3 (the metric's workload); "
                          "shock_vortex: Mach-2 shock/vortex 256x128x128 (learned form); "
-                         "hit: isotropic turbulence 512^3 (combined form)")
+                         "hit: isotropic turbulence 512^3 (combined form); "
+                         "slab: one GPU's share of a 512^3 block cut in 8 slabs, 512x512x64")
     ap.add_argument("--size", type=int, default=0, help="N (default: the workload's own)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-size", type=int, default=256)
@@ -241,7 +242,7 @@ def main():
     from cbns3d_cuda_amd import cases
 
     wl = args.workload
-    N = args.size or {"tgv": 256, "shock_vortex": 256, "hit": 512}[wl]
+    N = args.size or {"tgv": 256, "shock_vortex": 256, "hit": 512, "slab": 512}[wl]
     if wl == "tgv":
         case = cases.taylor_green(N)
         check = case
@@ -251,6 +252,11 @@ def main():
         check = case
         desc = ("Mach-2 shock / compressible vortex %dx%dx%d, inviscid, clustered curvilinear "
                 "mesh" % (N, N // 2, N // 2))
+    elif wl == "slab":
+        case = cases.smooth_box(N, N, N // 8, viscous=1)
+        check = cases.smooth_box(128, 128, 16, viscous=1)
+        desc = ("one GPU's share of a %d^3 viscous block in 8 slabs: %dx%dx%d, periodic "
+                "(ghost layers stand in for the halo exchange)" % (N, N, N, N // 8))
     else:
         case = cases.isotropic_turbulence(N, seed=0)
         check = cases.isotropic_turbulence(128, seed=0)
