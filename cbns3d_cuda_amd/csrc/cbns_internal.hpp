@@ -105,14 +105,18 @@ __device__ inline bool box_node3(const Box& b, int& i, int& j, int& k) {
 constexpr int N_XCD = CBNS_N_XCD;
 struct TileMarch {
   Box box;
-  int ntx, nty, nkc, chunk, rows;
+  int ntx, nty, nkc, chunk, rows, xh;
 };
-// rows: workgroup rows along j (the workgroup is 64 x rows threads)
-inline TileMarch tile_march(const Box& b, int chunk, int rows = BOX_Y) {
+// rows: workgroup rows along j (the workgroup is 64 x rows threads).
+// xh > 0: each wavefront (one row) also holds the xh nodes on either side of
+// its tile along i, so tiles are 64 - 2 xh nodes wide and the tile's
+// xi-stencils read those neighbours by lane shuffles (tile_march_lane).
+inline TileMarch tile_march(const Box& b, int chunk, int rows = BOX_Y, int xh = 0) {
   TileMarch t;
   t.box = b;
   t.rows = rows;
-  t.ntx = (b.n[0] + BOX_X - 1) / BOX_X;
+  t.xh = xh;
+  t.ntx = (b.n[0] + BOX_X - 2 * xh - 1) / (BOX_X - 2 * xh);
   t.nty = (b.n[1] + rows - 1) / rows;
   t.nkc = (b.n[2] + chunk - 1) / chunk;
   t.chunk = chunk;
@@ -131,6 +135,29 @@ __device__ inline bool tile_march_node(const TileMarch& t, int& i, int& j, int& 
   const unsigned ty = r % t.nty, kc = r / t.nty;
   const int ti = tx * BOX_X + threadIdx.x, tj = ty * t.rows + threadIdx.y;
   if (ti >= t.box.n[0] || tj >= t.box.n[1]) return false;
+  i = t.box.lo[0] + ti;
+  j = t.box.lo[1] + tj;
+  k0 = t.box.lo[2] + (int)kc * t.chunk;
+  k1 = min(k0 + t.chunk, t.box.lo[2] + t.box.n[2]);
+  return true;
+}
+// The lane of a tile march with xi-halo xh: returns false (for the whole
+// wavefront) past the box along j or past the tile count.  i is the lane's
+// node, lo[0] - xh <= i; `has`: i < lo[0] + n[0] + xh (the node exists and
+// its values are loaded); `own`: the node is in the tile and in the box.
+__device__ inline bool tile_march_lane(const TileMarch& t, int& i, int& j, int& k0, int& k1,
+                                       bool& has, bool& own) {
+  const unsigned nb = (unsigned)t.ntx * t.nty * t.nkc;
+  const unsigned per = (nb + N_XCD - 1) / N_XCD;
+  const unsigned lb = (blockIdx.x % N_XCD) * per + blockIdx.x / N_XCD;
+  if (lb >= nb) return false;
+  const unsigned tx = lb % t.ntx, r = lb / t.ntx;
+  const unsigned ty = r % t.nty, kc = r / t.nty;
+  const int lane = (int)threadIdx.x;
+  const int ti = (int)tx * (BOX_X - 2 * t.xh) - t.xh + lane, tj = ty * t.rows + threadIdx.y;
+  if (tj >= t.box.n[1]) return false;
+  has = ti < t.box.n[0] + t.xh;
+  own = lane >= t.xh && lane < BOX_X - t.xh && ti < t.box.n[0];
   i = t.box.lo[0] + ti;
   j = t.box.lo[1] + tj;
   k0 = t.box.lo[2] + (int)kc * t.chunk;

@@ -17,6 +17,14 @@
 // register window, so each plane value is loaded once per chunk instead of
 // five times (a plane of one field is 0.56 MB at 256^3: the +-2 planes of
 // four fields do not fit a 4 MB XCD L2 and would be re-read from HBM).
+// In the flux pass a wavefront holds its 60-node tile along i and two nodes
+// on either side; the xi-derivatives take their neighbours from the adjacent
+// lanes by wavefront shuffles (LDS crossbar) instead of 16 loads per node.
+// The flux pass waits on the vector-memory pipeline (address and data units
+// ~90 % busy at 256^3), not on HBM bytes: remapping its tiles so that
+// neighbouring rows share an XCD's L2 fetched 28 % fewer bytes and ran no
+// faster.  (The divergence pass keeps its loads: there the shuffles cost 30
+// VGPRs, two of its six waves per SIMD and 20 % of its time.)
 #include "cbns_internal.hpp"
 
 namespace cbns {
@@ -55,6 +63,33 @@ __device__ __forceinline__ double dgrad(const Consts& c, int a, int m, const dou
          1.0 / 12.0 * f[2 * st];
 }
 
+// xi-neighbours of a lane's value: n = v at i-2, i-1, i+1, i+2 (meaningful
+// on the lanes that own their node; the shuffles wrap at the wave's ends)
+template <class V>
+__device__ __forceinline__ void xi_neighbours(V v, V n[4]) {
+  const int lane = threadIdx.x;
+  n[0] = __shfl(v, lane - 2);
+  n[1] = __shfl(v, lane - 1);
+  n[2] = __shfl(v, lane + 1);
+  n[3] = __shfl(v, lane + 2);
+}
+
+// d/dxi of v at node i from the neighbouring lanes (every lane takes part);
+// the one-sided layers of a reference axis read memory (dgrad, from f at the
+// node)
+__device__ __forceinline__ double dxi(const Consts& c, int i, double v,
+                                      const double* __restrict__ f) {
+  double nb[4];
+  xi_neighbours(v, nb);
+  const double d =
+      (1.0 / 12.0) * nb[0] - 2.0 / 3.0 * nb[1] + (2.0 / 3.0) * nb[2] - 1.0 / 12.0 * nb[3];
+  // (only the four outer nodes -2, -1, IM, IM+1 of the viscous range: the
+  // halo lanes beyond them hold no node of the range)
+  if (!periodic(c, 0) && (i == -2 || i == -1 || i == c.IM || i == c.IM + 1))
+    return dgrad(c, 0, i, f, 1);
+  return d;
+}
+
 // 4th-order central difference from a 5-plane window (same arithmetic as
 // dgrad's central branch)
 __device__ __forceinline__ double dwin(const double w[5]) {
@@ -63,10 +98,12 @@ __device__ __forceinline__ double dwin(const double w[5]) {
 
 // T = double (plain form) or float (precision form: the index derivatives
 // are differenced in double and rounded, stresses, heat flux and fluxes are
-// formed and stored in float).  win[f][s]: field f (u,v,w,T) at k-2+s.
+// formed and stored in float).  win[f][s]: field f (u,v,w,T) at k-2+s;
+// dx[f]: d/dxi of field f.
 template <class T>
 __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, int k,
                                                  const double win[4][5],
+                                                 const double dx[4],
                                                  const double* __restrict__ prim,
                                                  const double* __restrict__ met,
                                                  double* __restrict__ mu,
@@ -77,7 +114,7 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
   const double* V = prim + P_V * S + o;
   const double* W = prim + P_W * S + o;
   const double* TT = prim + P_T * S + o;
-  const int sx = 1, sy = c.PX, sz = c.PXY;
+  const int sy = c.PX, sz = c.PXY;
   // zeta: the register window, except at the one-sided outer layers of a
   // reference axis
   const bool zc = periodic(c, 2) || (k >= 0 && k <= c.KM - 1);
@@ -85,10 +122,10 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
   const double vz = zc ? dwin(win[1]) : dgrad(c, 2, k, V, sz);
   const double wz = zc ? dwin(win[2]) : dgrad(c, 2, k, W, sz);
   const double tz = zc ? dwin(win[3]) : dgrad(c, 2, k, TT, sz);
-  T u_xi_l = (T)dgrad(c, 0, i, U, sx), u_eta_l = (T)dgrad(c, 1, j, U, sy), u_zeta_l = (T)uz;
-  const T v_xi_l = (T)dgrad(c, 0, i, V, sx), v_eta_l = (T)dgrad(c, 1, j, V, sy), v_zeta_l = (T)vz;
-  const T w_xi_l = (T)dgrad(c, 0, i, W, sx), w_eta_l = (T)dgrad(c, 1, j, W, sy), w_zeta_l = (T)wz;
-  const T T_xi = (T)dgrad(c, 0, i, TT, sx), T_eta = (T)dgrad(c, 1, j, TT, sy), T_zeta = (T)tz;
+  T u_xi_l = (T)dx[0], u_eta_l = (T)dgrad(c, 1, j, U, sy), u_zeta_l = (T)uz;
+  const T v_xi_l = (T)dx[1], v_eta_l = (T)dgrad(c, 1, j, V, sy), v_zeta_l = (T)vz;
+  const T w_xi_l = (T)dx[2], w_eta_l = (T)dgrad(c, 1, j, W, sy), w_zeta_l = (T)wz;
+  const T T_xi = (T)dx[3], T_eta = (T)dgrad(c, 1, j, TT, sy), T_zeta = (T)tz;
 
   // src/viscous_terms_kernel.cu:59-105
   const double T_l = win[3][2];
@@ -138,7 +175,8 @@ __global__ void __launch_bounds__(64 * VROWS) k_visc_fluxes(Consts c, TileMarch 
                                                      double* __restrict__ mu,
                                                      T* __restrict__ vf) {
   int i, j, k0, k1;
-  if (!tile_march_node(tm, i, j, k0, k1)) return;
+  bool has, own;
+  if (!tile_march_lane(tm, i, j, k0, k1, has, own)) return;
   const long S = c.S;
   const int sz = c.PXY;
   const double* f0[4] = {prim + P_U * S, prim + P_V * S, prim + P_W * S, prim + P_T * S};
@@ -147,15 +185,17 @@ __global__ void __launch_bounds__(64 * VROWS) k_visc_fluxes(Consts c, TileMarch 
 #pragma unroll
   for (int f = 0; f < 4; f++)
 #pragma unroll
-    for (int s = 0; s < 4; s++) win[f][s + 1] = f0[f][base + (k0 - 2 + s) * sz];
+    for (int s = 0; s < 4; s++) win[f][s + 1] = has ? f0[f][base + (k0 - 2 + s) * sz] : 0.0;
   for (int k = k0; k < k1; k++) {
+    double dx[4];
 #pragma unroll
     for (int f = 0; f < 4; f++) {
 #pragma unroll
       for (int s = 0; s < 4; s++) win[f][s] = win[f][s + 1];
-      win[f][4] = f0[f][base + (k + 2) * sz];
+      win[f][4] = has ? f0[f][base + (k + 2) * sz] : 0.0;
+      dx[f] = dxi(c, i, win[f][2], f0[f] + base + k * sz);
     }
-    visc_fluxes_node<T>(c, i, j, k, win, prim, met, mu, vf);
+    if (own) visc_fluxes_node<T>(c, i, j, k, win, dx, prim, met, mu, vf);
   }
 }
 
@@ -225,7 +265,7 @@ int viscous(cbns_block* b, T* vf, double* mu, double* out, int mode) {
   }
   {
     ProfScope ps(b, CBNS_K_VISC_FLUX);
-    const TileMarch tm = tile_march(vb, VISC_CHUNK, VROWS);
+    const TileMarch tm = tile_march(vb, VISC_CHUNK, VROWS, 2);
     k_visc_fluxes<T><<<tile_march_blocks(tm), dim3(BOX_X, VROWS), 0, b->stream>>>(c, tm, b->prim,
                                                                           b->met, mu, vf);
     CBNS_HIP(b, hipGetLastError());
