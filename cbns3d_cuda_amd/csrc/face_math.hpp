@@ -7,8 +7,12 @@
 //     dot products, rows 3..4 are the tangent projections;
 //   - R (right eigenvectors) is applied in closed form;
 //   - each WENO5 evaluation uses one division (weights multiplied through by
-//     IS1^2 IS2^2 IS3^2) instead of seven, and the left and right
-//     evaluations of a field share their two common smoothness terms;
+//     IS1^2 IS2^2 IS3^2, the linear weights and the 1/6 of the candidate
+//     stencils folded into the candidates and the normaliser) instead of
+//     seven; the left and right evaluations of a field share their two
+//     common smoothness terms and, in double, one reciprocal;
+//   - the Lax-Friedrichs flux reuses 1/rho, |u|^2/2 and E of the
+//     reconstructed states (the reference re-derives E from p);
 //   - 1/x is formed once and multiplied where the reference divides by the
 //     same x repeatedly;
 //   - reciprocals and reciprocal square roots start from the hardware
@@ -141,39 +145,59 @@ __device__ __forceinline__ void char_project(const Basis<T>& B, const T q[5], T 
   W[4] = -B.qm * q[0] + B.mx * q[1] + B.my * q[2] + B.mz * q[3];
 }
 
-// q = R W, then to primitives (src/weno.cu:106-118)
+// A reconstructed face state: primitives plus 1/rho, |u|^2/2 and the total
+// energy E they were derived from (p = gam1 (E - rho |u|^2/2)).
 template <class T>
-__device__ __forceinline__ void char_to_prim(const GasT<T>& g, const Basis<T>& B, const T W[5],
-                                             T& r, T& u, T& v, T& w, T& p) {
-  const T s = W[0] + W[2];
-  const T dl = B.c * (W[2] - W[0]);
-  const T rho = s + W[1];
-  const T m1 = B.uu * rho + B.nx * dl + B.lx * W[3] + B.mx * W[4];
-  const T m2 = B.vv * rho + B.ny * dl + B.ly * W[3] + B.my * W[4];
-  const T m3 = B.ww * rho + B.nz * dl + B.lz * W[3] + B.mz * W[4];
-  const T E = B.H * s + T(0.5) * B.q2 * W[1] + B.qn * dl + B.ql * W[3] + B.qm * W[4];
-  const T ir = fast_rcp(rho);
-  r = rho;
-  u = m1 * ir;
-  v = m2 * ir;
-  w = m3 * ir;
-  p = g.gam1 * (E - T(0.5) * rho * (u * u + v * v + w * w));
+struct Side {
+  T r, u, v, w, p, ir, k, E;
+};
+
+// conservative -> face state (src/weno.cu:113-118 / src/primitive_bc_kernel.cu:63-67)
+template <class T>
+__device__ __forceinline__ Side<T> q_to_side(const GasT<T>& g, const T q[5]) {
+  Side<T> s;
+  s.r = q[0];
+  s.ir = fast_rcp(s.r);
+  s.u = q[1] * s.ir;
+  s.v = q[2] * s.ir;
+  s.w = q[3] * s.ir;
+  s.k = T(0.5) * (s.u * s.u + s.v * s.v + s.w * s.w);
+  s.E = q[4];
+  s.p = g.gam1 * (s.E - s.r * s.k);
+  return s;
 }
 
-// src/lf_flux.cu:22-78 (dissipation speed |sum_d max_LR(|u_d|+c) n_d|)
+// q = R W, then to the face state (src/weno.cu:106-118)
 template <class T>
-__device__ __forceinline__ void lf_flux(const GasT<T>& g, T rR, T uR, T vR, T wR, T pR, T rL,
-                                        T uL, T vL, T wL, T pL, T nx, T ny, T nz, T f[5]) {
-  const T aR = g.gamma * pR * fast_rcp(rR);
+__device__ __forceinline__ Side<T> char_to_side(const GasT<T>& g, const Basis<T>& B,
+                                                const T W[5]) {
+  const T s = W[0] + W[2];
+  const T dl = B.c * (W[2] - W[0]);
+  T q[5];
+  q[0] = s + W[1];
+  q[1] = B.uu * q[0] + B.nx * dl + B.lx * W[3] + B.mx * W[4];
+  q[2] = B.vv * q[0] + B.ny * dl + B.ly * W[3] + B.my * W[4];
+  q[3] = B.ww * q[0] + B.nz * dl + B.lz * W[3] + B.mz * W[4];
+  q[4] = B.H * s + T(0.5) * B.q2 * W[1] + B.qn * dl + B.ql * W[3] + B.qm * W[4];
+  return q_to_side(g, q);
+}
+
+// src/lf_flux.cu:22-78 (dissipation speed |sum_d max_LR(|u_d|+c) n_d|).
+// The total energies are the reconstructed E (the reference forms
+// p/gam1 + rho |u|^2/2, which equals E up to rounding).
+template <class T>
+__device__ __forceinline__ void lf_flux(const GasT<T>& g, const Side<T>& R, const Side<T>& L,
+                                        T nx, T ny, T nz, T f[5]) {
+  const T rR = R.r, uR = R.u, vR = R.v, wR = R.w, pR = R.p;
+  const T rL = L.r, uL = L.u, vL = L.v, wL = L.w, pL = L.p;
+  const T aR = g.gamma * pR * R.ir;
   const T cR = aR * fast_rsq(aR);
-  const T kR = T(0.5) * (uR * uR + vR * vR + wR * wR);
-  const T HR = aR * g.gam1_inv + kR;
-  const T EtR = pR * g.gam1_inv + rR * kR;
-  const T aL = g.gamma * pL * fast_rcp(rL);
+  const T HR = aR * g.gam1_inv + R.k;
+  const T EtR = R.E;
+  const T aL = g.gamma * pL * L.ir;
   const T cL = aL * fast_rsq(aL);
-  const T kL = T(0.5) * (uL * uL + vL * vL + wL * wL);
-  const T HL = aL * g.gam1_inv + kL;
-  const T EtL = pL * g.gam1_inv + rL * kL;
+  const T HL = aL * g.gam1_inv + L.k;
+  const T EtL = L.E;
   const T qR = uR * nx + vR * ny + wR * nz;
   const T qL = uL * nx + vL * ny + wL * nz;
   const T s0 = fmax(fabs(uR) + cR, fabs(uL) + cL);
@@ -204,38 +228,49 @@ __device__ __forceinline__ void q_to_prim(const GasT<T>& g, const T q[5], T& r, 
 // w0..w5 (nodes m-2..m+3): left = weno5(w0..w4), right = weno5(w5..w1).
 // The (13/12)(.)^2 terms of the two middle sub-stencils are the same
 // second differences on both sides and are formed once.
+//
+// With s_k = IS_k^2 the JS weights are d_k / s_k, d = (0.1, 0.6, 0.3), and
+// the candidates f_k / 6 with f = (2a-7b+11c, -b+5c+2d, 2c+5d-e).  Scaling
+// numerator and normaliser by s1 s2 s3 / 0.6:
+//   value = (s2 s3 f1/6 + s1 s3 f2 + s1 s2 f3/2) / (s2 s3 + 6 s1 s3 + 3 s1 s2).
+template <class T>
+__device__ __forceinline__ void weno5_side(T a, T b, T c, T d, T e, T ta, T tb, T& num,
+                                           T& den) {
+  // ta, tb: (13/12) (b - 2c + d)^2 and (13/12) (c - 2d + e)^2
+  const T eps = T(1.0e-6), k13 = T(13.0 / 12.0), q = T(0.25);
+  const T d1 = a - T(2) * b + c, d2 = a - T(4) * b + T(3) * c;
+  const T d4 = b - d, d6 = T(3) * c - T(4) * d + e;
+  const T IS1 = k13 * d1 * d1 + q * d2 * d2 + eps;
+  const T IS2 = ta + q * d4 * d4 + eps;
+  const T IS3 = tb + q * d6 * d6 + eps;
+  const T s1 = IS1 * IS1, s2 = IS2 * IS2, s3 = IS3 * IS3;
+  const T p1 = s2 * s3, p2 = s1 * s3, p3 = s1 * s2;
+  const T g1 = T(1.0 / 3.0) * a - T(7.0 / 6.0) * b + T(11.0 / 6.0) * c;  // f1 / 6
+  const T g2 = -b + T(5) * c + T(2) * d;                                   // f2
+  const T g3 = c + T(2.5) * d - T(0.5) * e;                                // f3 / 2
+  num = p1 * g1 + p2 * g2 + p3 * g3;
+  den = p1 + T(6) * p2 + T(3) * p3;
+}
+
 template <class T>
 __device__ __forceinline__ void weno5_pair(T w0, T w1, T w2, T w3, T w4, T w5, T& left,
                                            T& right) {
-  const T eps = T(1.0e-6), k13 = T(13.0 / 12.0), q = T(0.25);
+  const T k13 = T(13.0 / 12.0);
   const T dc = w1 - T(2) * w2 + w3;  // centred at m
   const T dd = w2 - T(2) * w3 + w4;  // centred at m+1
   const T tc = k13 * dc * dc, td = k13 * dd * dd;
-  {
-    const T d1 = w0 - T(2) * w1 + w2, d2 = w0 - T(4) * w1 + T(3) * w2;
-    const T d4 = w1 - w3, d6 = T(3) * w2 - T(4) * w3 + w4;
-    const T IS1 = k13 * d1 * d1 + q * d2 * d2 + eps;
-    const T IS2 = tc + q * d4 * d4 + eps;
-    const T IS3 = td + q * d6 * d6 + eps;
-    const T s1 = IS1 * IS1, s2 = IS2 * IS2, s3 = IS3 * IS3;
-    const T a1 = T(0.1) * (s2 * s3), a2 = T(0.6) * (s1 * s3), a3 = T(0.3) * (s1 * s2);
-    const T f1 = T(2) * w0 - T(7) * w1 + T(11) * w2;
-    const T f2 = -w1 + T(5) * w2 + T(2) * w3;
-    const T f3 = T(2) * w2 + T(5) * w3 - w4;
-    left = (a1 * f1 + a2 * f2 + a3 * f3) * fast_rcp(T(6) * (a1 + a2 + a3));
-  }
-  {
-    const T d1 = w5 - T(2) * w4 + w3, d2 = w5 - T(4) * w4 + T(3) * w3;
-    const T d4 = w4 - w2, d6 = T(3) * w3 - T(4) * w2 + w1;
-    const T IS1 = k13 * d1 * d1 + q * d2 * d2 + eps;
-    const T IS2 = td + q * d4 * d4 + eps;
-    const T IS3 = tc + q * d6 * d6 + eps;
-    const T s1 = IS1 * IS1, s2 = IS2 * IS2, s3 = IS3 * IS3;
-    const T a1 = T(0.1) * (s2 * s3), a2 = T(0.6) * (s1 * s3), a3 = T(0.3) * (s1 * s2);
-    const T f1 = T(2) * w5 - T(7) * w4 + T(11) * w3;
-    const T f2 = -w4 + T(5) * w3 + T(2) * w2;
-    const T f3 = T(2) * w3 + T(5) * w2 - w1;
-    right = (a1 * f1 + a2 * f2 + a3 * f3) * fast_rcp(T(6) * (a1 + a2 + a3));
+  T nl, dl, nr, dr;
+  weno5_side(w0, w1, w2, w3, w4, tc, td, nl, dl);
+  weno5_side(w5, w4, w3, w2, w1, td, tc, nr, dr);
+  if constexpr (sizeof(T) == sizeof(double)) {
+    // one reciprocal for both sides: dl, dr lie in [1e-24, 1e120] for any
+    // smoothness indicator below 1e30, so dl dr stays in range
+    const T x = fast_rcp(dl * dr);
+    left = nl * (dr * x);
+    right = nr * (dl * x);
+  } else {
+    left = nl * fast_rcp(dl);
+    right = nr * fast_rcp(dr);
   }
 }
 
@@ -273,10 +308,8 @@ __device__ __forceinline__ void face_flux(const GasT<T>& g, const T q[6][5], con
   for (int s = 0; s < 6; s++)
     w[s] = -B.qm * q[s][0] + B.mx * q[s][1] + B.my * q[s][2] + B.mz * q[s][3];
   weno5_pair(w[0], w[1], w[2], w[3], w[4], w[5], WL[4], WR[4]);
-  T rL, uL, vL, wL, pL, rR, uR, vR, wR, pR;
-  char_to_prim(g, B, WL, rL, uL, vL, wL, pL);
-  char_to_prim(g, B, WR, rR, uR, vR, wR, pR);
-  lf_flux(g, rR, uR, vR, wR, pR, rL, uL, vL, wL, pL, nx, ny, nz, f);
+  const Side<T> L = char_to_side(g, B, WL), R = char_to_side(g, B, WR);
+  lf_flux(g, R, L, nx, ny, nz, f);
 }
 
 // Precision form of face_flux: the Roe basis, the characteristic projection
@@ -344,10 +377,7 @@ __device__ __forceinline__ void face_flux_mixed(const GasT<T>& gt, const GasT<do
       out[side][4] = q[2][4] + (double)E;
     }
   }
-  double rL, uL, vL, wL, pL, rR, uR, vR, wR, pR;
-  q_to_prim(g, qL, rL, uL, vL, wL, pL);
-  q_to_prim(g, qR, rR, uR, vR, wR, pR);
-  lf_flux(g, rR, uR, vR, wR, pR, rL, uL, vL, wL, pL, nx, ny, nz, f);
+  lf_flux(g, q_to_side(g, qR), q_to_side(g, qL), nx, ny, nz, f);
 }
 
 // face flux of the chosen arithmetic form: F = double (plain), float (precision)

@@ -162,10 +162,7 @@ __device__ __forceinline__ void face_linear(const GasT<double>& g, const double 
       qR[c] = q[2][c] + (double)((F(2) * d5 - F(13) * d4 + F(47) * d3 - F(3) * d1) * F(1.0 / 60.0));
     }
   }
-  double rL, uL, vL, wL, pL, rR, uR, vR, wR, pR;
-  q_to_prim(g, qL, rL, uL, vL, wL, pL);
-  q_to_prim(g, qR, rR, uR, vR, wR, pR);
-  lf_flux(g, rR, uR, vR, wR, pR, rL, uL, vL, wL, pL, nx, ny, nz, f);
+  lf_flux(g, q_to_side(g, qR), q_to_side(g, qL), nx, ny, nz, f);
 }
 
 // Logit of this lane's face.  Every lane of the wavefront must call it
