@@ -84,10 +84,11 @@ struct Basis {
   T K, o;                                // gam1/(2c^2), 0.5/c
 };
 
+// From the two node states as (rho, u, v, w, H).
 template <class T>
-__device__ __forceinline__ void roe_basis(const GasT<T>& g, T rL, T uL, T vL, T wL, T pL,
-                                          T rR, T uR, T vR, T wR, T pR, T n_x, T n_y, T n_z,
-                                          Basis<T>& B) {
+__device__ __forceinline__ void roe_basis_h(const GasT<T>& g, T rL, T uL, T vL, T wL, T HL,
+                                            T rR, T uR, T vR, T wR, T HR, T n_x, T n_y, T n_z,
+                                            Basis<T>& B) {
   T l_x = T(0), l_y = n_z, l_z = -n_y;
   T m_x, m_y, m_z;
   // |l| > 1e-8 of the reference, compared squared
@@ -107,13 +108,10 @@ __device__ __forceinline__ void roe_basis(const GasT<T>& g, T rL, T uL, T vL, T 
   B.nx = n_x * inv_n; B.ny = n_y * inv_n; B.nz = n_z * inv_n;
   const T inv_l = fast_rsq(l2);
   B.lx = l_x * inv_l; B.ly = l_y * inv_l; B.lz = l_z * inv_l;
-  const T inv_m = fast_rsq(m_x * m_x + m_y * m_y + m_z * m_z);
+  // |m|^2 = |l|^2 |n|^2 for either choice of tangents
+  const T inv_m = inv_l * inv_n;
   B.mx = m_x * inv_m; B.my = m_y * inv_m; B.mz = m_z * inv_m;
 
-  // H = c^2/gam1 + |u|^2/2 with c^2 = gamma p / rho
-  const T irL = fast_rcp(rL), irR = fast_rcp(rR);
-  const T HL = g.gamma * pL * irL * g.gam1_inv + T(0.5) * (uL * uL + vL * vL + wL * wL);
-  const T HR = g.gamma * pR * irR * g.gam1_inv + T(0.5) * (uR * uR + vR * vR + wR * wR);
   const T ratio = rR * fast_rsq(rR * rL);  // sqrt(rR / rL)
   const T inv1r = fast_rcp(T(1) + ratio);
   B.uu = (uL + ratio * uR) * inv1r;
@@ -129,6 +127,17 @@ __device__ __forceinline__ void roe_basis(const GasT<T>& g, T rL, T uL, T vL, T 
   B.qm = B.uu * B.mx + B.vv * B.my + B.ww * B.mz;
   B.o = T(0.5) * inv_c;
   B.K = T(0.5) * g.gam1 * inv_c * inv_c;
+}
+
+// From (rho, u, v, w, p): H = c^2/gam1 + |u|^2/2 with c^2 = gamma p / rho.
+template <class T>
+__device__ __forceinline__ void roe_basis(const GasT<T>& g, T rL, T uL, T vL, T wL, T pL,
+                                          T rR, T uR, T vR, T wR, T pR, T n_x, T n_y, T n_z,
+                                          Basis<T>& B) {
+  const T irL = fast_rcp(rL), irR = fast_rcp(rR);
+  const T HL = g.gamma * pL * irL * g.gam1_inv + T(0.5) * (uL * uL + vL * vL + wL * wL);
+  const T HR = g.gamma * pR * irR * g.gam1_inv + T(0.5) * (uR * uR + vR * vR + wR * wR);
+  roe_basis_h(g, rL, uL, vL, wL, HL, rR, uR, vR, wR, HR, n_x, n_y, n_z, B);
 }
 
 // W = L q for one node (q = rho, rho u, rho v, rho w, E).
@@ -165,6 +174,20 @@ __device__ __forceinline__ Side<T> q_to_side(const GasT<T>& g, const T q[5]) {
   s.E = q[4];
   s.p = g.gam1 * (s.E - s.r * s.k);
   return s;
+}
+
+// primitives -> face state
+template <class T>
+__device__ __forceinline__ Side<T> prim_to_side(const GasT<T>& g, T r, T u, T v, T w, T p) {
+  Side<T> s{r, u, v, w, p, fast_rcp(r), T(0.5) * (u * u + v * v + w * w), T(0)};
+  s.E = p * g.gam1_inv + r * s.k;
+  return s;
+}
+
+// total enthalpy of a face state, as roe_basis forms it
+template <class T>
+__device__ __forceinline__ T side_h(const GasT<T>& g, const Side<T>& s) {
+  return g.gamma * s.p * s.ir * g.gam1_inv + s.k;
 }
 
 // q = R W, then to the face state (src/weno.cu:106-118)
@@ -236,13 +259,15 @@ __device__ __forceinline__ void q_to_prim(const GasT<T>& g, const T q[5], T& r, 
 template <class T>
 __device__ __forceinline__ void weno5_side(T a, T b, T c, T d, T e, T ta, T tb, T& num,
                                            T& den) {
-  // ta, tb: (13/12) (b - 2c + d)^2 and (13/12) (c - 2d + e)^2
-  const T eps = T(1.0e-6), k13 = T(13.0 / 12.0), q = T(0.25);
-  const T d1 = a - T(2) * b + c, d2 = a - T(4) * b + T(3) * c;
-  const T d4 = b - d, d6 = T(3) * c - T(4) * d + e;
-  const T IS1 = k13 * d1 * d1 + q * d2 * d2 + eps;
-  const T IS2 = ta + q * d4 * d4 + eps;
-  const T IS3 = tb + q * d6 * d6 + eps;
+  // ta, tb: (13/12) (b - 2c + d)^2 + eps and (13/12) (c - 2d + e)^2 + eps
+  const T eps = T(1.0e-6), k13 = T(13.0 / 12.0);
+  const T d1 = a - T(2) * b + c;
+  const T h2 = T(0.5) * a - T(2) * b + T(1.5) * c;  // (a - 4b + 3c) / 2
+  const T h4 = T(0.5) * b - T(0.5) * d;             // (b - d) / 2
+  const T h6 = T(1.5) * c - T(2) * d + T(0.5) * e;  // (3c - 4d + e) / 2
+  const T IS1 = k13 * d1 * d1 + (h2 * h2 + eps);
+  const T IS2 = ta + h4 * h4;
+  const T IS3 = tb + h6 * h6;
   const T s1 = IS1 * IS1, s2 = IS2 * IS2, s3 = IS3 * IS3;
   const T p1 = s2 * s3, p2 = s1 * s3, p3 = s1 * s2;
   const T g1 = T(1.0 / 3.0) * a - T(7.0 / 6.0) * b + T(11.0 / 6.0) * c;  // f1 / 6
@@ -255,10 +280,10 @@ __device__ __forceinline__ void weno5_side(T a, T b, T c, T d, T e, T ta, T tb, 
 template <class T>
 __device__ __forceinline__ void weno5_pair(T w0, T w1, T w2, T w3, T w4, T w5, T& left,
                                            T& right) {
-  const T k13 = T(13.0 / 12.0);
+  const T k13 = T(13.0 / 12.0), eps = T(1.0e-6);
   const T dc = w1 - T(2) * w2 + w3;  // centred at m
   const T dd = w2 - T(2) * w3 + w4;  // centred at m+1
-  const T tc = k13 * dc * dc, td = k13 * dd * dd;
+  const T tc = k13 * dc * dc + eps, td = k13 * dd * dd + eps;
   T nl, dl, nr, dr;
   weno5_side(w0, w1, w2, w3, w4, tc, td, nl, dl);
   weno5_side(w5, w4, w3, w2, w1, td, tc, nr, dr);
@@ -274,16 +299,16 @@ __device__ __forceinline__ void weno5_pair(T w0, T w1, T w2, T w3, T w4, T w5, T
   }
 }
 
-// Full face: q[s][c] holds nodes m-2..m+3 (face between m and m+1); pl, pr
-// are the primitives of nodes m and m+1; (nx,ny,nz) the unnormalised face
+// Full face: q[s][c] holds nodes m-2..m+3 (face between m and m+1); sl, sr
+// are the states of nodes m and m+1; (nx,ny,nz) the unnormalised face
 // vector 0.5 (J grad xi |_m + J grad xi |_{m+1}).  Characteristic fields are
 // reconstructed one at a time so only six projected values are live.
 template <class T>
-__device__ __forceinline__ void face_flux(const GasT<T>& g, const T q[6][5], const T pl[5],
-                                          const T pr[5], T nx, T ny, T nz, T f[5]) {
+__device__ __forceinline__ void face_flux(const GasT<T>& g, const T q[6][5], const Side<T>& sl,
+                                          const Side<T>& sr, T nx, T ny, T nz, T f[5]) {
   Basis<T> B;
-  roe_basis(g, pl[0], pl[1], pl[2], pl[3], pl[4], pr[0], pr[1], pr[2], pr[3], pr[4], nx, ny, nz,
-            B);
+  roe_basis_h(g, sl.r, sl.u, sl.v, sl.w, side_h(g, sl), sr.r, sr.u, sr.v, sr.w, side_h(g, sr), nx,
+              ny, nz, B);
   // rows 0..2 of L q share A = K (E - u.m + q2/2 rho) and Bn = o (qn rho - n.m)
   T A[6], Bn[6];
 #pragma unroll
@@ -321,12 +346,12 @@ __device__ __forceinline__ void face_flux(const GasT<T>& g, const T q[6][5], con
 // float cancellation error.
 template <class T>
 __device__ __forceinline__ void face_flux_mixed(const GasT<T>& gt, const GasT<double>& g,
-                                                const double q[6][5], const double pl[5],
-                                                const double pr[5], double nx, double ny,
+                                                const double q[6][5], const Side<double>& sl,
+                                                const Side<double>& sr, double nx, double ny,
                                                 double nz, double f[5]) {
   Basis<T> B;
-  roe_basis(gt, (T)pl[0], (T)pl[1], (T)pl[2], (T)pl[3], (T)pl[4], (T)pr[0], (T)pr[1], (T)pr[2],
-            (T)pr[3], (T)pr[4], (T)nx, (T)ny, (T)nz, B);
+  roe_basis_h(gt, (T)sl.r, (T)sl.u, (T)sl.v, (T)sl.w, (T)side_h(g, sl), (T)sr.r, (T)sr.u,
+              (T)sr.v, (T)sr.w, (T)side_h(g, sr), (T)nx, (T)ny, (T)nz, B);
   T d[6][5];
 #pragma unroll
   for (int s = 0; s < 6; s++)
@@ -383,13 +408,13 @@ __device__ __forceinline__ void face_flux_mixed(const GasT<T>& gt, const GasT<do
 // face flux of the chosen arithmetic form: F = double (plain), float (precision)
 template <class F>
 __device__ __forceinline__ void face_flux_form(const GasT<double>& g, const double q[6][5],
-                                               const double pl[5], const double pr[5], double nx,
-                                               double ny, double nz, double f[5]) {
+                                               const Side<double>& sl, const Side<double>& sr,
+                                               double nx, double ny, double nz, double f[5]) {
   if constexpr (sizeof(F) == sizeof(double)) {
-    face_flux<double>(g, q, pl, pr, nx, ny, nz, f);
+    face_flux<double>(g, q, sl, sr, nx, ny, nz, f);
   } else {
     const GasT<F> gt{(F)g.gamma, (F)g.gam1, (F)g.gam1_inv};
-    face_flux_mixed<F>(gt, g, q, pl, pr, nx, ny, nz, f);
+    face_flux_mixed<F>(gt, g, q, sl, sr, nx, ny, nz, f);
   }
 }
 

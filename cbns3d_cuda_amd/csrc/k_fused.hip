@@ -65,10 +65,7 @@ __device__ __forceinline__ void stage_face(const GasT<double>& g, const double q
                                            double nx, double ny, double nz, bool troubled,
                                            double f[5]) {
   if (!HYB || troubled) {
-    double pl[5], pr[5];
-    q_to_prim(g, q[2], pl[0], pl[1], pl[2], pl[3], pl[4]);
-    q_to_prim(g, q[3], pr[0], pr[1], pr[2], pr[3], pr[4]);
-    face_flux_form<F>(g, q, pl, pr, nx, ny, nz, f);
+    face_flux_form<F>(g, q, q_to_side(g, q[2]), q_to_side(g, q[3]), nx, ny, nz, f);
   } else {
     face_linear<F>(g, q, nx, ny, nz, f);
   }
