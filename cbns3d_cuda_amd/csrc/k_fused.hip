@@ -50,6 +50,11 @@ constexpr int CHUNK = MARCH_CHUNK;  // nodes per thread along the march axis
 // form, -6 % for FP64).
 template <class F>
 constexpr int xi_axis() { return sizeof(F) == sizeof(double) ? 1 : 2; }
+// nodes a march wavefront updates along i (see k_march)
+template <int AX, class F>
+constexpr int march_width() {
+  return !(FUSE_XI && AX == xi_axis<F>()) ? MARCH_X : (AX == 2 ? MARCH_X - 6 : MARCH_X - 1);
+}
 
 template <class T>
 __device__ __forceinline__ void load_q(const double* __restrict__ Q, long S, int o, T q[5]) {
@@ -115,11 +120,18 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
   constexpr bool XI = AX == xi_axis<F>() && FUSE_XI;
   const int ax_t = AX == 1 ? 2 : 1;  // transverse axis other than i
   const int lane = threadIdx.x;
-  // XI: a wavefront covers 63 nodes; lane 0 sits one node to the left
-  const int i = XI ? upd_lo(c, 0) - 1 + blockIdx.x * (MARCH_X - 1) + lane
-                   : upd_lo(c, 0) + blockIdx.x * MARCH_X + lane;
+  // XI in the eta march: a wavefront updates 63 nodes, its lane 0 sits one
+  // node to the left and loads its xi-stencils.  XI in the zeta march (XSH):
+  // the xi-stencils come from the neighbouring lanes' march windows by
+  // wavefront shuffles, so a wavefront holds 3 nodes on either side of the
+  // 58 it updates (the zeta march is bound by its memory instructions, the
+  // eta march by arithmetic).
+  constexpr bool XSH = XI && AX == 2;
+  constexpr int XL = XSH ? 3 : (XI ? 1 : 0);  // lanes left of the first updated node
+  const int i = upd_lo(c, 0) - XL + blockIdx.x * march_width<AX, F>() + lane;
   const int t = upd_lo(c, ax_t) + blockIdx.y * MARCH_Y + threadIdx.y;
-  if (i > upd_hi(c, 0) || t > upd_hi(c, ax_t)) return;
+  if (i > upd_hi(c, 0) + (XSH ? 3 : 0) || t > upd_hi(c, ax_t)) return;
+  const bool own = lane >= XL && lane < XL + march_width<AX, F>() && i <= upd_hi(c, 0);
   const int mlo = upd_lo(c, AX), mhi = upd_hi(c, AX);
   const int m0 = mlo + blockIdx.z * CHUNK;
   if (m0 > mhi) return;
@@ -163,14 +175,14 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
     load_q(Q, S, o + 3 * st, q[5]);
     // the zeta sweep's update operands are fetched before the face work so
     // their latency overlaps it
-    double rin[5], qnv[5], jo = 0.0;
+    const double jo = jb;  // J at node m, loaded as the previous face's right node
+    double rin[5], qnv[5];
     if (AX == 2) {
 #pragma unroll
       for (int n = 0; n < 5; n++) {
         rin[n] = sa.rhs[n * S + o];
         qnv[n] = sa.stage > 1 ? sa.Qn[n * S + o] : 0.0;
       }
-      jo = J[o];
     }
     ax = bx; ay = by; az = bz;
     jb = J[o + st];
@@ -178,22 +190,39 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
     double f[5], fxr[5];
     if (XI) {
       // the xi-face between node (i, m) and (i+1, m): stencil i-2 .. i+3
-      double qx[6][5];
-#pragma unroll
-      for (int s = 0; s < 6; s++) {
-        if (s == 2) {
-#pragma unroll
-          for (int n = 0; n < 5; n++) qx[2][n] = q[2][n];
-        } else {
-          load_q(Q, S, o + (s - 2), qx[s]);
-        }
-      }
-      const double Ja = J[o], Jb = J[o + 1];
+      double qx[6][5], nxx, nxy, nxz;
+      bool tx = false;
       const double* __restrict__ xm = met + XI_X * S;
-      const double nxx = 0.5 * (xm[o] * Ja + xm[o + 1] * Jb);
-      const double nxy = 0.5 * (xm[S + o] * Ja + xm[S + o + 1] * Jb);
-      const double nxz = 0.5 * (xm[2 * S + o] * Ja + xm[2 * S + o + 1] * Jb);
-      const bool tx = HYB && (sa.flags_x[o - 1] | sa.flags_x[o] | sa.flags_x[o + 1]);
+      if constexpr (XSH) {
+        // every lane takes part in the shuffles
+#pragma unroll
+        for (int s = 0; s < 6; s++)
+#pragma unroll
+          for (int n = 0; n < 5; n++) qx[s][n] = s == 2 ? q[2][n] : __shfl(q[2][n], lane + s - 2);
+        const double jx = xm[o] * jo, jy = xm[S + o] * jo, jz = xm[2 * S + o] * jo;
+        nxx = 0.5 * (jx + __shfl(jx, lane + 1));
+        nxy = 0.5 * (jy + __shfl(jy, lane + 1));
+        nxz = 0.5 * (jz + __shfl(jz, lane + 1));
+        if (HYB) {
+          const int fo = sa.flags_x[o];
+          tx = (__shfl(fo, lane - 1) | fo | __shfl(fo, lane + 1)) != 0;
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < 6; s++) {
+          if (s == 2) {
+#pragma unroll
+            for (int n = 0; n < 5; n++) qx[2][n] = q[2][n];
+          } else {
+            load_q(Q, S, o + (s - 2), qx[s]);
+          }
+        }
+        const double Ja = jo, Jb = J[o + 1];
+        nxx = 0.5 * (xm[o] * Ja + xm[o + 1] * Jb);
+        nxy = 0.5 * (xm[S + o] * Ja + xm[S + o + 1] * Jb);
+        nxz = 0.5 * (xm[2 * S + o] * Ja + xm[2 * S + o + 1] * Jb);
+        tx = HYB && (sa.flags_x[o - 1] | sa.flags_x[o] | sa.flags_x[o + 1]);
+      }
       stage_face<F, HYB>(g, qx, nxx, nxy, nxz, tx, fxr);
     }
     if (HYB) {
@@ -210,7 +239,7 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
     if (AX == 1 && XI) {
 #pragma unroll
       for (int n = 0; n < 5; n++)
-        if (lane > 0) sa.rhs[n * S + o] = dfx[n] + (f[n] - fprev[n]);
+        if (own) sa.rhs[n * S + o] = dfx[n] + (f[n] - fprev[n]);
     } else if (AX == 1 && FUSE_XI) {  // xi-faces in the zeta march
 #pragma unroll
       for (int n = 0; n < 5; n++) sa.rhs[n * S + o] = f[n] - fprev[n];
@@ -234,11 +263,11 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
         if (sa.stage == 1) out = qin - sa.dt * df;
         else if (sa.stage == 2) out = 0.75 * qnv[n] + 0.25 * (qin - sa.dt * df);
         else out = (qnv[n] + 2.0 * (qin - sa.dt * df)) * (1.0 / 3.0);
-        if (!XI || lane > 0) sa.Qout[n * S + o] = out;
+        if (own) sa.Qout[n * S + o] = out;
         qn_out[n] = out;
       }
       // primitives of the new state (src/primitive_bc_kernel.cu:63-76)
-      if (!XI || lane > 0) {
+      if (own) {
         double rr, uu, vv, ww, pp;
         q_to_prim(g, qn_out, rr, uu, vv, ww, pp);
         sa.prim[o] = rr;
@@ -292,7 +321,7 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
   {
     const int nt = upd_hi(c, 2) - upd_lo(c, 2) + 1;
     const int nm = upd_hi(c, 1) - upd_lo(c, 1) + 1;
-    const int per_wave = FUSE_XI && xi_axis<F>() == 1 ? MARCH_X - 1 : MARCH_X;
+    const int per_wave = march_width<1, F>();
     dim3 grid((ni + per_wave - 1) / per_wave, (nt + MARCH_Y - 1) / MARCH_Y,
               (nm + CHUNK - 1) / CHUNK);
     ProfScope ps(b, CBNS_K_MARCH_ETA);
@@ -307,7 +336,7 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
   {
     const int nt = upd_hi(c, 1) - upd_lo(c, 1) + 1;
     const int nm = upd_hi(c, 2) - upd_lo(c, 2) + 1;
-    const int per_wave = FUSE_XI && xi_axis<F>() == 2 ? MARCH_X - 1 : MARCH_X;
+    const int per_wave = march_width<2, F>();
     dim3 grid((ni + per_wave - 1) / per_wave, (nt + MARCH_Y - 1) / MARCH_Y,
               (nm + CHUNK - 1) / CHUNK);
     ProfScope ps(b, CBNS_K_MARCH_ZETA);
