@@ -9,8 +9,9 @@
 //   - each WENO5 evaluation uses one division (weights multiplied through by
 //     IS1^2 IS2^2 IS3^2, the linear weights and the 1/6 of the candidate
 //     stencils folded into the candidates and the normaliser) instead of
-//     seven; the left and right evaluations of a field share their two
-//     common smoothness terms and, in double, one reciprocal;
+//     seven, on first differences relative to the centre value; the left
+//     and right evaluations of a field share their differences, two
+//     smoothness terms and, in double, one reciprocal;
 //   - the Lax-Friedrichs flux reuses 1/rho, |u|^2/2 and E of the
 //     reconstructed states (the reference re-derives E from p);
 //   - 1/x is formed once and multiplied where the reference divides by the
@@ -154,11 +155,12 @@ __device__ __forceinline__ void char_project(const Basis<T>& B, const T q[5], T 
   W[4] = -B.qm * q[0] + B.mx * q[1] + B.my * q[2] + B.mz * q[3];
 }
 
-// A reconstructed face state: primitives plus 1/rho, |u|^2/2 and the total
-// energy E they were derived from (p = gam1 (E - rho |u|^2/2)).
+// A reconstructed face state: primitives plus 1/rho, |u|^2/2 and the
+// conservative state they were derived from (momenta, total energy E;
+// p = gam1 (E - rho |u|^2/2)).
 template <class T>
 struct Side {
-  T r, u, v, w, p, ir, k, E;
+  T r, u, v, w, p, ir, k, E, ru, rv, rw;
 };
 
 // conservative -> face state (src/weno.cu:113-118 / src/primitive_bc_kernel.cu:63-67)
@@ -173,13 +175,17 @@ __device__ __forceinline__ Side<T> q_to_side(const GasT<T>& g, const T q[5]) {
   s.k = T(0.5) * (s.u * s.u + s.v * s.v + s.w * s.w);
   s.E = q[4];
   s.p = g.gam1 * (s.E - s.r * s.k);
+  s.ru = q[1];
+  s.rv = q[2];
+  s.rw = q[3];
   return s;
 }
 
 // primitives -> face state
 template <class T>
 __device__ __forceinline__ Side<T> prim_to_side(const GasT<T>& g, T r, T u, T v, T w, T p) {
-  Side<T> s{r, u, v, w, p, fast_rcp(r), T(0.5) * (u * u + v * v + w * w), T(0)};
+  Side<T> s{r, u, v, w, p, fast_rcp(r), T(0.5) * (u * u + v * v + w * w), T(0), r * u, r * v,
+            r * w};
   s.E = p * g.gam1_inv + r * s.k;
   return s;
 }
@@ -205,34 +211,33 @@ __device__ __forceinline__ Side<T> char_to_side(const GasT<T>& g, const Basis<T>
   return q_to_side(g, q);
 }
 
-// src/lf_flux.cu:22-78 (dissipation speed |sum_d max_LR(|u_d|+c) n_d|).
-// The total energies are the reconstructed E (the reference forms
-// p/gam1 + rho |u|^2/2, which equals E up to rounding).
+// src/lf_flux.cu:22-78 (dissipation speed |sum_d max_LR(|u_d|+c) n_d|),
+//   f = (F(L).n + F(R).n - alpha (Q_R - Q_L)) / 2,  F(Q).n = (rho q_n, rho q_n u + p n, ...,
+//   rho q_n H),
+// on the reconstructed conservative states: the mass fluxes are (rho u).n
+// and the jumps are those of rho u and E (the reference re-forms rho u and
+// E from the primitives, which equals them up to rounding).
 template <class T>
 __device__ __forceinline__ void lf_flux(const GasT<T>& g, const Side<T>& R, const Side<T>& L,
                                         T nx, T ny, T nz, T f[5]) {
-  const T rR = R.r, uR = R.u, vR = R.v, wR = R.w, pR = R.p;
-  const T rL = L.r, uL = L.u, vL = L.v, wL = L.w, pL = L.p;
-  const T aR = g.gamma * pR * R.ir;
+  const T aR = g.gamma * R.p * R.ir;
   const T cR = aR * fast_rsq(aR);
   const T HR = aR * g.gam1_inv + R.k;
-  const T EtR = R.E;
-  const T aL = g.gamma * pL * L.ir;
+  const T aL = g.gamma * L.p * L.ir;
   const T cL = aL * fast_rsq(aL);
   const T HL = aL * g.gam1_inv + L.k;
-  const T EtL = L.E;
-  const T qR = uR * nx + vR * ny + wR * nz;
-  const T qL = uL * nx + vL * ny + wL * nz;
-  const T s0 = fmax(fabs(uR) + cR, fabs(uL) + cL);
-  const T s1 = fmax(fabs(vR) + cR, fabs(vL) + cL);
-  const T s2 = fmax(fabs(wR) + cR, fabs(wL) + cL);
-  const T a = fabs(s0 * nx + s1 * ny + s2 * nz);
-  const T mL = rL * qL, mR = rR * qR;
-  f[0] = T(0.5) * (mL + mR - a * (rR - rL));
-  f[1] = T(0.5) * (mL * uL + pL * nx + mR * uR + pR * nx - a * (rR * uR - rL * uL));
-  f[2] = T(0.5) * (mL * vL + pL * ny + mR * vR + pR * ny - a * (rR * vR - rL * vL));
-  f[3] = T(0.5) * (mL * wL + pL * nz + mR * wR + pR * nz - a * (rR * wR - rL * wL));
-  f[4] = T(0.5) * (mL * HL + mR * HR - a * (EtR - EtL));
+  const T s0 = fmax(fabs(R.u) + cR, fabs(L.u) + cL);
+  const T s1 = fmax(fabs(R.v) + cR, fabs(L.v) + cL);
+  const T s2 = fmax(fabs(R.w) + cR, fabs(L.w) + cL);
+  const T ha = T(0.5) * fabs(s0 * nx + s1 * ny + s2 * nz);
+  const T hmL = T(0.5) * (L.ru * nx + L.rv * ny + L.rw * nz);  // rho q_n / 2
+  const T hmR = T(0.5) * (R.ru * nx + R.rv * ny + R.rw * nz);
+  const T hp = T(0.5) * (L.p + R.p);
+  f[0] = hmL + hmR - ha * (R.r - L.r);
+  f[1] = hmL * L.u + hmR * R.u + hp * nx - ha * (R.ru - L.ru);
+  f[2] = hmL * L.v + hmR * R.v + hp * ny - ha * (R.rv - L.rv);
+  f[3] = hmL * L.w + hmR * R.w + hp * nz - ha * (R.rw - L.rw);
+  f[4] = hmL * HL + hmR * HR - ha * (R.E - L.E);
 }
 
 // conservative -> primitive (src/primitive_bc_kernel.cu:63-67)
@@ -249,30 +254,38 @@ __device__ __forceinline__ void q_to_prim(const GasT<T>& g, const T q[5], T& r, 
 
 // WENO5-JS on both sides of one face from the 6 projected stencil values
 // w0..w5 (nodes m-2..m+3): left = weno5(w0..w4), right = weno5(w5..w1).
-// The (13/12)(.)^2 terms of the two middle sub-stencils are the same
-// second differences on both sides and are formed once.
 //
-// With s_k = IS_k^2 the JS weights are d_k / s_k, d = (0.1, 0.6, 0.3), and
-// the candidates f_k / 6 with f = (2a-7b+11c, -b+5c+2d, 2c+5d-e).  Scaling
-// numerator and normaliser by s1 s2 s3 / 0.6:
-//   value = (s2 s3 f1/6 + s1 s3 f2 + s1 s2 f3/2) / (s2 s3 + 6 s1 s3 + 3 s1 s2).
+// Written on the first differences e_k = w_{k+1} - w_k, which the two sides
+// share (the right side reads them reversed and negated), and relative to
+// the centre value.  With s_k = IS_k^2 the JS weights are d_k / s_k,
+// d = (0.1, 0.6, 0.3), and the candidates f_k / 6; scaling numerator and
+// normaliser by s1 s2 s3 / 0.6 and taking the centre c out,
+//   value = c + (p1 g1 + p2 g2 + p3 g3) / (p1 + 6 p2 + 3 p3),
+//   p1 = s2 s3, p2 = s1 s3, p3 = s1 s2,
+//   g1 = f1/6 - c = 5/6 E1 - 1/3 E0,  g2 = f2 - 6c = E1 + 2 E2,
+//   g3 = f3/2 - 3c = 2 E2 - E3/2,
+// with E0..E3 = b-a, c-b, d-c, e-d of the side's stencil a..e.  The
+// smoothness indicators use IS1 = 13/12 (E1-E0)^2 + (0.5 (E1-E0) + E1)^2,
+// IS2 = 13/12 (E2-E1)^2 + (0.5 (E1+E2))^2, IS3 = 13/12 (E3-E2)^2 +
+// (0.5 (E3-E2) - E2)^2 (each + eps), the same polynomials as
+// src/weno.cu:19-42 regrouped; the (13/12)(.)^2 + eps terms of the two
+// middle sub-stencils are the same second differences on both sides.
 template <class T>
-__device__ __forceinline__ void weno5_side(T a, T b, T c, T d, T e, T ta, T tb, T& num,
+__device__ __forceinline__ void weno5_side(T E0, T E1, T E2, T E3, T d1, T ta, T tb, T& num,
                                            T& den) {
-  // ta, tb: (13/12) (b - 2c + d)^2 + eps and (13/12) (c - 2d + e)^2 + eps
+  // d1 = E1 - E0; ta = 13/12 (E2-E1)^2 + eps; tb = 13/12 (E3-E2)^2 + eps
   const T eps = T(1.0e-6), k13 = T(13.0 / 12.0);
-  const T d1 = a - T(2) * b + c;
-  const T h2 = T(0.5) * a - T(2) * b + T(1.5) * c;  // (a - 4b + 3c) / 2
-  const T h4 = T(0.5) * b - T(0.5) * d;             // (b - d) / 2
-  const T h6 = T(1.5) * c - T(2) * d + T(0.5) * e;  // (3c - 4d + e) / 2
+  const T h2 = T(0.5) * d1 + E1;
+  const T h4 = T(0.5) * (E1 + E2);
+  const T h6 = T(0.5) * (E3 - E2) - E2;
   const T IS1 = k13 * d1 * d1 + (h2 * h2 + eps);
   const T IS2 = ta + h4 * h4;
   const T IS3 = tb + h6 * h6;
   const T s1 = IS1 * IS1, s2 = IS2 * IS2, s3 = IS3 * IS3;
   const T p1 = s2 * s3, p2 = s1 * s3, p3 = s1 * s2;
-  const T g1 = T(1.0 / 3.0) * a - T(7.0 / 6.0) * b + T(11.0 / 6.0) * c;  // f1 / 6
-  const T g2 = -b + T(5) * c + T(2) * d;                                   // f2
-  const T g3 = c + T(2.5) * d - T(0.5) * e;                                // f3 / 2
+  const T g1 = T(5.0 / 6.0) * E1 - T(1.0 / 3.0) * E0;
+  const T g2 = E1 + T(2) * E2;
+  const T g3 = T(2) * E2 - T(0.5) * E3;
   num = p1 * g1 + p2 * g2 + p3 * g3;
   den = p1 + T(6) * p2 + T(3) * p3;
 }
@@ -281,21 +294,22 @@ template <class T>
 __device__ __forceinline__ void weno5_pair(T w0, T w1, T w2, T w3, T w4, T w5, T& left,
                                            T& right) {
   const T k13 = T(13.0 / 12.0), eps = T(1.0e-6);
-  const T dc = w1 - T(2) * w2 + w3;  // centred at m
-  const T dd = w2 - T(2) * w3 + w4;  // centred at m+1
+  const T e01 = w1 - w0, e12 = w2 - w1, e23 = w3 - w2, e34 = w4 - w3, e45 = w5 - w4;
+  const T dc = e23 - e12;  // second difference centred at m
+  const T dd = e34 - e23;  // centred at m+1
   const T tc = k13 * dc * dc + eps, td = k13 * dd * dd + eps;
   T nl, dl, nr, dr;
-  weno5_side(w0, w1, w2, w3, w4, tc, td, nl, dl);
-  weno5_side(w5, w4, w3, w2, w1, td, tc, nr, dr);
+  weno5_side(e01, e12, e23, e34, e12 - e01, tc, td, nl, dl);
+  weno5_side(-e45, -e34, -e23, -e12, e45 - e34, td, tc, nr, dr);
   if constexpr (sizeof(T) == sizeof(double)) {
     // one reciprocal for both sides: dl, dr lie in [1e-24, 1e120] for any
     // smoothness indicator below 1e30, so dl dr stays in range
     const T x = fast_rcp(dl * dr);
-    left = nl * (dr * x);
-    right = nr * (dl * x);
+    left = w2 + nl * (dr * x);
+    right = w3 + nr * (dl * x);
   } else {
-    left = nl * fast_rcp(dl);
-    right = nr * fast_rcp(dr);
+    left = w2 + nl * fast_rcp(dl);
+    right = w3 + nr * fast_rcp(dr);
   }
 }
 
