@@ -5,9 +5,12 @@
 //                registers: each node is loaded once per sweep and each face
 //                is computed once.  One of the two marches (xi_axis) also
 //                computes, at every node, the xi-face on its right; the left
-//                one comes from the lane below by a wavefront shuffle, so a
-//                wavefront updates 63 nodes and its lane 0 only supplies that
-//                face.  Writes rhs = dE + dF (or dF).
+//                one comes from the lane below by a wavefront shuffle.  In
+//                the eta march (FP64) a wavefront updates 63 nodes, its lane
+//                0 only supplies that face, and the xi-stencils are loaded;
+//                in the zeta march (float faces) they come from the
+//                neighbouring lanes' windows and a wavefront updates 58
+//                nodes (march_width).  Writes rhs = dE + dF (or dF).
 //                (FUSE_XI=0: a separate k_xfaces pass writes the xi-face
 //                fluxes to HBM and the march reads them back.)
 //   [viscous]    k_visc_fluxes + k_visc_div subtract the viscous divergence
