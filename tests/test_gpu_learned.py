@@ -204,8 +204,12 @@ def test_hybrid_extremes(oracle, bias, expect):
         b.free()
 
 
-def test_fp32_hybrid_stage_close_to_oracle(oracle):
-    case = cases.taylor_green(16)
+@pytest.mark.parametrize("builder", ["tgv", "box_rpp"])
+def test_fp32_hybrid_stage_close_to_oracle(oracle, builder):
+    # box_rpp: lines along i crossing the zeta march's 58-node wavefronts,
+    # reference xi axis
+    case = {"tgv": lambda: cases.taylor_green(16),
+            "box_rpp": lambda: cases.smooth_box(125, 7, 6, viscous=1, bc=(0, 1, 1))}[builder]()
     print("case", case.name, flush=True)
     P = random_params(12)
     o, b, dt = _stage_pair(oracle, case, P, 0.0, FORM_FP32 | FORM_HYBRID)

@@ -20,9 +20,21 @@ from helpers import G, oracle_block, rel_err_state
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("builder", ["vortex", "tgv"])
+FP32_CASES = {
+    "vortex": lambda: cases.isentropic_vortex(20),
+    "tgv": lambda: cases.taylor_green(16),
+    # lines along i crossing the zeta march's 58-node wavefronts (3 of them),
+    # periodic and reference xi axis; all-reference C-grid
+    "box_130x6x5_ppp_visc": lambda: cases.smooth_box(130, 6, 5, viscous=1),
+    "box_125x7x6_rpp_visc": lambda: cases.smooth_box(
+        125, 7, 6, viscous=1, bc=(cases.BC_REF, cases.BC_PERIODIC, cases.BC_PERIODIC)),
+    "cgrid_viscous": lambda: cases.c_grid_channel(viscous=1),
+}
+
+
+@pytest.mark.parametrize("builder", list(FP32_CASES))
 def test_fp32_stage_close_to_oracle(oracle, builder):
-    case = cases.isentropic_vortex(20) if builder == "vortex" else cases.taylor_green(16)
+    case = FP32_CASES[builder]()
     o = oracle_block(oracle, case)
     o.bc_primitive()
     if case.params.viscous:
