@@ -85,11 +85,11 @@ struct Basis {
   T K, o;                                // gam1/(2c^2), 0.5/c
 };
 
-// From the two node states as (rho, u, v, w, H).
+// Characteristic basis from the face normal and the Roe-averaged state
+// (u, v, w, H) (src/calc_eigenvectors_roe.cu:28-150).
 template <class T>
-__device__ __forceinline__ void roe_basis_h(const GasT<T>& g, T rL, T uL, T vL, T wL, T HL,
-                                            T rR, T uR, T vR, T wR, T HR, T n_x, T n_y, T n_z,
-                                            Basis<T>& B) {
+__device__ __forceinline__ void basis_from_roe(const GasT<T>& g, T uu, T vv, T ww, T H, T n_x,
+                                               T n_y, T n_z, Basis<T>& B) {
   T l_x = T(0), l_y = n_z, l_z = -n_y;
   T m_x, m_y, m_z;
   // |l| > 1e-8 of the reference, compared squared
@@ -113,12 +113,7 @@ __device__ __forceinline__ void roe_basis_h(const GasT<T>& g, T rL, T uL, T vL, 
   const T inv_m = inv_l * inv_n;
   B.mx = m_x * inv_m; B.my = m_y * inv_m; B.mz = m_z * inv_m;
 
-  const T ratio = rR * fast_rsq(rR * rL);  // sqrt(rR / rL)
-  const T inv1r = fast_rcp(T(1) + ratio);
-  B.uu = (uL + ratio * uR) * inv1r;
-  B.vv = (vL + ratio * vR) * inv1r;
-  B.ww = (wL + ratio * wR) * inv1r;
-  B.H = (HL + ratio * HR) * inv1r;
+  B.uu = uu; B.vv = vv; B.ww = ww; B.H = H;
   B.q2 = B.uu * B.uu + B.vv * B.vv + B.ww * B.ww;
   const T c2 = g.gam1 * (B.H - T(0.5) * B.q2);
   const T inv_c = fast_rsq(c2);
@@ -128,6 +123,34 @@ __device__ __forceinline__ void roe_basis_h(const GasT<T>& g, T rL, T uL, T vL, 
   B.qm = B.uu * B.mx + B.vv * B.my + B.ww * B.mz;
   B.o = T(0.5) * inv_c;
   B.K = T(0.5) * g.gam1 * inv_c * inv_c;
+}
+
+// From the two node states as (rho, u, v, w, H): Roe average with
+// r = sqrt(rho_R / rho_L).
+template <class T>
+__device__ __forceinline__ void roe_basis_h(const GasT<T>& g, T rL, T uL, T vL, T wL, T HL,
+                                            T rR, T uR, T vR, T wR, T HR, T n_x, T n_y, T n_z,
+                                            Basis<T>& B) {
+  const T ratio = rR * fast_rsq(rR * rL);  // sqrt(rR / rL)
+  const T inv1r = fast_rcp(T(1) + ratio);
+  basis_from_roe(g, (uL + ratio * uR) * inv1r, (vL + ratio * vR) * inv1r,
+                 (wL + ratio * wR) * inv1r, (HL + ratio * HR) * inv1r, n_x, n_y, n_z, B);
+}
+
+// From the two node states as conservative states: with a = 1/sqrt(rho),
+// the Roe average sum sqrt(rho) f / sum sqrt(rho) is sum a (rho f) / sum
+// sqrt(rho), and rho H = E + p.
+template <class T>
+__device__ __forceinline__ void roe_basis_q(const GasT<T>& g, const T qL[5], const T qR[5],
+                                            T n_x, T n_y, T n_z, Basis<T>& B) {
+  const T aL = fast_rsq(qL[0]), aR = fast_rsq(qR[0]);
+  const T D = fast_rcp(qL[0] * aL + qR[0] * aR);
+  const T wL = aL * D, wR = aR * D;
+  // p = gam1 (E - |rho u|^2 / (2 rho)), 1/rho = a^2
+  const T pL = g.gam1 * (qL[4] - T(0.5) * (qL[1] * qL[1] + qL[2] * qL[2] + qL[3] * qL[3]) * (aL * aL));
+  const T pR = g.gam1 * (qR[4] - T(0.5) * (qR[1] * qR[1] + qR[2] * qR[2] + qR[3] * qR[3]) * (aR * aR));
+  basis_from_roe(g, qL[1] * wL + qR[1] * wR, qL[2] * wL + qR[2] * wR, qL[3] * wL + qR[3] * wR,
+                 (qL[4] + pL) * wL + (qR[4] + pR) * wR, n_x, n_y, n_z, B);
 }
 
 // From (rho, u, v, w, p): H = c^2/gam1 + |u|^2/2 with c^2 = gamma p / rho.
@@ -178,15 +201,6 @@ __device__ __forceinline__ Side<T> q_to_side(const GasT<T>& g, const T q[5]) {
   s.ru = q[1];
   s.rv = q[2];
   s.rw = q[3];
-  return s;
-}
-
-// primitives -> face state
-template <class T>
-__device__ __forceinline__ Side<T> prim_to_side(const GasT<T>& g, T r, T u, T v, T w, T p) {
-  Side<T> s{r, u, v, w, p, fast_rcp(r), T(0.5) * (u * u + v * v + w * w), T(0), r * u, r * v,
-            r * w};
-  s.E = p * g.gam1_inv + r * s.k;
   return s;
 }
 
@@ -313,16 +327,14 @@ __device__ __forceinline__ void weno5_pair(T w0, T w1, T w2, T w3, T w4, T w5, T
   }
 }
 
-// Full face: q[s][c] holds nodes m-2..m+3 (face between m and m+1); sl, sr
-// are the states of nodes m and m+1; (nx,ny,nz) the unnormalised face
-// vector 0.5 (J grad xi |_m + J grad xi |_{m+1}).  Characteristic fields are
+// Full face: q[s][c] holds nodes m-2..m+3 (face between m and m+1); B is
+// the face's characteristic basis (face_flux: the Roe average of nodes m and
+// m+1); (nx,ny,nz) the unnormalised face vector
+// 0.5 (J grad xi |_m + J grad xi |_{m+1}).  Characteristic fields are
 // reconstructed one at a time so only six projected values are live.
 template <class T>
-__device__ __forceinline__ void face_flux(const GasT<T>& g, const T q[6][5], const Side<T>& sl,
-                                          const Side<T>& sr, T nx, T ny, T nz, T f[5]) {
-  Basis<T> B;
-  roe_basis_h(g, sl.r, sl.u, sl.v, sl.w, side_h(g, sl), sr.r, sr.u, sr.v, sr.w, side_h(g, sr), nx,
-              ny, nz, B);
+__device__ __forceinline__ void face_flux_basis(const GasT<T>& g, const T q[6][5],
+                                                const Basis<T>& B, T nx, T ny, T nz, T f[5]) {
   // rows 0..2 of L q share A = K (E - u.m + q2/2 rho) and Bn = o (qn rho - n.m)
   T A[6], Bn[6];
 #pragma unroll
@@ -349,6 +361,14 @@ __device__ __forceinline__ void face_flux(const GasT<T>& g, const T q[6][5], con
   weno5_pair(w[0], w[1], w[2], w[3], w[4], w[5], WL[4], WR[4]);
   const Side<T> L = char_to_side(g, B, WL), R = char_to_side(g, B, WR);
   lf_flux(g, R, L, nx, ny, nz, f);
+}
+
+template <class T>
+__device__ __forceinline__ void face_flux(const GasT<T>& g, const T q[6][5], T nx, T ny, T nz,
+                                          T f[5]) {
+  Basis<T> B;
+  roe_basis_q(g, q[2], q[3], nx, ny, nz, B);
+  face_flux_basis(g, q, B, nx, ny, nz, f);
 }
 
 // Precision form of face_flux: the Roe basis, the characteristic projection
@@ -422,13 +442,12 @@ __device__ __forceinline__ void face_flux_mixed(const GasT<T>& gt, const GasT<do
 // face flux of the chosen arithmetic form: F = double (plain), float (precision)
 template <class F>
 __device__ __forceinline__ void face_flux_form(const GasT<double>& g, const double q[6][5],
-                                               const Side<double>& sl, const Side<double>& sr,
                                                double nx, double ny, double nz, double f[5]) {
   if constexpr (sizeof(F) == sizeof(double)) {
-    face_flux<double>(g, q, sl, sr, nx, ny, nz, f);
+    face_flux<double>(g, q, nx, ny, nz, f);
   } else {
     const GasT<F> gt{(F)g.gamma, (F)g.gam1, (F)g.gam1_inv};
-    face_flux_mixed<F>(gt, g, q, sl, sr, nx, ny, nz, f);
+    face_flux_mixed<F>(gt, g, q, q_to_side(g, q[2]), q_to_side(g, q[3]), nx, ny, nz, f);
   }
 }
 

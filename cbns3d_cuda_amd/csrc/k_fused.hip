@@ -73,7 +73,7 @@ __device__ __forceinline__ void stage_face(const GasT<double>& g, const double q
                                            double nx, double ny, double nz, bool troubled,
                                            double f[5]) {
   if (!HYB || troubled) {
-    face_flux_form<F>(g, q, q_to_side(g, q[2]), q_to_side(g, q[3]), nx, ny, nz, f);
+    face_flux_form<F>(g, q, nx, ny, nz, f);
   } else {
     face_linear<F>(g, q, nx, ny, nz, f);
   }

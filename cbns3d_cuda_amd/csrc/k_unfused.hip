@@ -37,8 +37,11 @@ __global__ void __launch_bounds__(256) k_face_flux(Consts c, Box box, int d,
     pr[n] = prim[n * S + bb];
   }
   const GasT<double> g{c.gamma, c.gam1, c.gam1_inv};
-  face_flux<double>(g, q, prim_to_side(g, pl[0], pl[1], pl[2], pl[3], pl[4]),
-                    prim_to_side(g, pr[0], pr[1], pr[2], pr[3], pr[4]), nx, ny, nz, f);
+  // the Roe average from the node primitives, as the reference reads them
+  Basis<double> B;
+  roe_basis(g, pl[0], pl[1], pl[2], pl[3], pl[4], pr[0], pr[1], pr[2], pr[3], pr[4], nx, ny, nz,
+            B);
+  face_flux_basis<double>(g, q, B, nx, ny, nz, f);
 #pragma unroll
   for (int n = 0; n < 5; n++) flux[(d * 5 + n) * S + a] = f[n];
 }
