@@ -125,6 +125,7 @@ int ensure_stage_scratch(cbns_block* b) {
   int rc;
   if (fused_stage_uses_face_flux() && (rc = lazy_alloc(b, &b->flux, 15 * S))) return rc;
   if (b->c.viscous && (rc = lazy_alloc(b, &b->vtmp, 12 * S))) return rc;
+  if (b->c.viscous && (rc = lazy_alloc(b, &b->lam_part, visc_flux_blocks(b->c)))) return rc;
   return CBNS_OK;
 }
 
@@ -214,7 +215,7 @@ void cbns_destroy(cbns_block* b) {
   (void)hipSetDevice(b->device);
   if (b->stream) (void)hipStreamSynchronize(b->stream);
   double* ps[] = {b->met, b->Q, b->qp, b->qspare, b->prim, b->mu, b->rhs, b->flux, b->dfv, b->vtmp,
-                  b->red};
+                  b->red, b->lam_part};
   for (double* p : ps)
     if (p) (void)hipFree(p);
   for (void* p : {(void*)b->flags, (void*)b->ind_P, b->ind_pack, (void*)b->ind_count})
@@ -445,6 +446,27 @@ int cbns_rk3_step(cbns_block* b, double t_end, double* t_cur, double* dt_out) {
   if (rc) return rc;
   if (!t_cur) return fail(b, CBNS_ERR_ARG, "null t_cur");
   double dt = 0.0;
+  if (b->c.viscous) {
+    // the first stage's viscous flux pass reads every node's primitives and
+    // metrics anyway: it also forms the step's time step on the device (the
+    // same per-node term and maximum as cbns_calc_dt, limited to the rest of
+    // the run), which the stages read there; dt reaches the host once, after
+    // the step.  A NaN step leaves the state untouched and is reported.
+    if ((rc = ensure_stage_scratch(b))) return rc;
+    b->dt_fold = true;
+    b->dt_cap = t_end - *t_cur;
+    for (int s = 1; s <= 3 && !rc; s++) rc = cbns_rk3_stage(b, 0.0, s);
+    b->dt_fold = false;
+    if (rc) return rc;
+    CBNS_HIP(b, hipMemcpyAsync(b->h_scalar + 2, b->red + 2, 2 * sizeof(double),
+                               hipMemcpyDeviceToHost, b->stream));
+    CBNS_HIP(b, hipStreamSynchronize(b->stream));
+    dt = b->h_scalar[2];
+    if (std::isnan(dt)) return fail(b, CBNS_ERR_NAN, "An error occurred while calculating the time step.");
+    *t_cur += dt;
+    if (dt_out) *dt_out = dt;
+    return CBNS_OK;
+  }
   if ((rc = cbns_calc_dt(b, &dt))) return rc;
   if (t_end - *t_cur < dt) dt = t_end - *t_cur;
   *t_cur += dt;

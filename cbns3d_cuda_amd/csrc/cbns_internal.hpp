@@ -54,6 +54,56 @@ __host__ __device__ inline int upd_hi(const Consts& c, int a) {
 enum { XI_X = 0, XI_Y, XI_Z, ETA_X, ETA_Y, ETA_Z, ZETA_X, ZETA_Y, ZETA_Z, JAC };
 enum { P_RHO = 0, P_U, P_V, P_W, P_P, P_T };
 
+// Spectral-radius sum of node o, src/time_step_kernel.cu:43-94, one
+// rounding per operation (no contraction) wherever it is inlined, so the
+// time step matches the oracle bit for bit.
+__device__ inline double node_lambda(const Consts& c, const double* __restrict__ prim,
+                                     const double* __restrict__ met, int o) {
+#pragma clang fp contract(off)
+  const long S = c.S;
+  const double rr = prim[o], uu = prim[S + o], vv = prim[2 * S + o], ww = prim[3 * S + o];
+  const double pp = prim[4 * S + o];
+  const double cs = sqrt(c.gamma * pp / rr);
+  double factor_v = 0.0;
+  if (c.viscous) {
+    const double T_l = prim[5 * S + o];
+    const double mu = (1.0 + c.C_T_inf) / (T_l + c.C_T_inf) * sqrt(T_l * T_l * T_l);
+    factor_v = 2.0 * mu * c.C_dt_v * sqrt(uu * uu + vv * vv + ww * ww) / cs / rr;
+  }
+  double sum = 0.0;
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double n_x = met[(3 * d + 0) * S + o], n_y = met[(3 * d + 1) * S + o],
+                 n_z = met[(3 * d + 2) * S + o];
+    const double qn = uu * n_x + vv * n_y + ww * n_z;
+    const double n_abs = sqrt(n_x * n_x + n_y * n_y + n_z * n_z);
+    double tt = fabs(qn) + cs * n_abs;
+    if (c.viscous) tt += n_abs * n_abs * factor_v;
+    sum = d == 0 ? tt : sum + tt;
+  }
+  return sum;
+}
+
+// Maximum over a wavefront (every lane takes part) and over a workgroup of
+// `threads` threads (tid: the linear thread index; the result is valid in
+// wavefront 0); lds holds threads/64 doubles.
+__device__ inline double wave_max(double v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
+  return v;
+}
+__device__ inline double block_max(double v, double* lds, int tid, int threads) {
+  v = wave_max(v);
+  const int lane = tid & 63, wid = tid >> 6;
+  if (lane == 0) lds[wid] = v;
+  __syncthreads();
+  if (wid == 0) {
+    v = lane < threads / 64 ? lds[lane] : -INFINITY;
+    v = wave_max(v);
+  }
+  return v;
+}
+
 // A 3-D box of nodes [lo, lo+n).
 struct Box {
   int lo[3];
@@ -194,7 +244,12 @@ struct cbns_block {
   double* flux = nullptr;    // 15*S (lazy: reference-decomposed calls)
   double* dfv = nullptr;     // 4*S  (lazy)
   double* vtmp = nullptr;    // 12*S (lazy): viscous fluxes Ev,Fv,Gv
-  double* red = nullptr;     // reduction scratch
+  double* red = nullptr;     // reduction scratch: [0..1] calc_dt, [2..3] folded dt, [4..] partials
+  double* lam_part = nullptr;  // (lazy, viscous) spectral-radius maxima of the flux pass
+  // cbns_rk3_step of a viscous block: stage 1's viscous flux pass also forms
+  // the step's dt (limited to dt_cap) on the device, where the stages read it
+  bool dt_fold = false;
+  double dt_cap = 0.0;
   int red_n = 0;
   double* h_scalar = nullptr;  // pinned host scalars [4]
   // learned hybrid (cbns_set_indicator)
@@ -247,8 +302,12 @@ int launch_conservative_all(cbns_block* b, double* Q);
 int launch_freestream(cbns_block* b);
 int launch_numerical_flux(cbns_block* b, const double* Q);
 int launch_viscous_unfused(cbns_block* b);
-int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode);
-int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs);
+int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode,
+                   double* lam_part = nullptr);
+int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs, double* lam_part = nullptr);
+long visc_flux_blocks(const Consts& c);
+int launch_dt_from_partials(cbns_block* b, const double* partial, int n, double cap,
+                            double* out);
 int launch_update_rk3(cbns_block* b, double dt, int stage);
 int launch_calc_dt(cbns_block* b, double* dt_host);
 int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout);

@@ -13,68 +13,32 @@ namespace {
 
 constexpr int TPB = 256;
 
-__device__ inline double wave_max(double v) {
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_xor(v, off, 64));
-  return v;
-}
-
-__device__ inline double block_max(double v, double* lds) {
-  v = wave_max(v);
-  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
-  if (lane == 0) lds[wid] = v;
-  __syncthreads();
-  if (wid == 0) {
-    v = lane < (TPB / 64) ? lds[lane] : -INFINITY;
-    v = wave_max(v);
-  }
-  return v;
-}
-
 __global__ void __launch_bounds__(TPB) k_time_step(Consts c, const double* __restrict__ prim,
                                                    const double* __restrict__ met,
                                                    double* __restrict__ partial) {
   __shared__ double lds[TPB / 64];
   const long N = (long)c.IM * c.JM * c.KM;
-  const long S = c.S;
   double lam = -INFINITY;
   int nan_seen = 0;
   for (long t = (long)blockIdx.x * TPB + threadIdx.x; t < N; t += (long)gridDim.x * TPB) {
     const int i = (int)(t % c.IM);
     const int j = (int)((t / c.IM) % c.JM);
     const int k = (int)(t / ((long)c.IM * c.JM));
-    const int o = idx(c, i, j, k);
-    const double rr = prim[o], uu = prim[S + o], vv = prim[2 * S + o], ww = prim[3 * S + o];
-    const double pp = prim[4 * S + o];
-    const double cs = sqrt(c.gamma * pp / rr);
-    double factor_v = 0.0;
-    if (c.viscous) {
-      const double T_l = prim[5 * S + o];
-      const double mu = (1.0 + c.C_T_inf) / (T_l + c.C_T_inf) * sqrt(T_l * T_l * T_l);
-      factor_v = 2.0 * mu * c.C_dt_v * sqrt(uu * uu + vv * vv + ww * ww) / cs / rr;
-    }
-    double sum = 0.0;
-#pragma unroll
-    for (int d = 0; d < 3; d++) {
-      const double n_x = met[(3 * d + 0) * S + o], n_y = met[(3 * d + 1) * S + o],
-                   n_z = met[(3 * d + 2) * S + o];
-      const double qn = uu * n_x + vv * n_y + ww * n_z;
-      const double n_abs = sqrt(n_x * n_x + n_y * n_y + n_z * n_z);
-      double tt = fabs(qn) + cs * n_abs;
-      if (c.viscous) tt += n_abs * n_abs * factor_v;
-      sum = d == 0 ? tt : sum + tt;
-    }
+    const double sum = node_lambda(c, prim, met, idx(c, i, j, k));
     if (isnan(sum)) nan_seen = 1;
     else lam = fmax(lam, sum);
   }
-  lam = block_max(lam, lds);
+  lam = block_max(lam, lds, threadIdx.x, TPB);
   __syncthreads();
   const int any_nan = __syncthreads_or(nan_seen);
   if (threadIdx.x == 0) partial[blockIdx.x] = any_nan ? NAN : lam;
 }
 
+// out[0] = CFL / max, limited to cap (the rest of the run, t_end - t_cur;
+// src/Block3d.cpp:241-246), out[1] = max; both NaN if any partial is NaN
 __global__ void __launch_bounds__(TPB) k_dt_final(const double* __restrict__ partial, int n,
-                                                  double CFL, double* __restrict__ out) {
+                                                  double CFL, double cap,
+                                                  double* __restrict__ out) {
   __shared__ double lds[TPB / 64];
   double lam = -INFINITY;
   int nan_seen = 0;
@@ -83,11 +47,12 @@ __global__ void __launch_bounds__(TPB) k_dt_final(const double* __restrict__ par
     if (isnan(v)) nan_seen = 1;
     else lam = fmax(lam, v);
   }
-  lam = block_max(lam, lds);
+  lam = block_max(lam, lds, threadIdx.x, TPB);
   __syncthreads();
   const int any_nan = __syncthreads_or(nan_seen);
   if (threadIdx.x == 0) {
-    out[0] = any_nan ? NAN : CFL / lam;
+    const double dt = CFL / lam;
+    out[0] = any_nan ? NAN : (cap < dt ? cap : dt);
     out[1] = any_nan ? NAN : lam;
   }
 }
@@ -101,12 +66,22 @@ int launch_calc_dt(cbns_block* b, double* dt_host) {
   if (nb > b->red_n - 4) nb = b->red_n - 4;
   k_time_step<<<nb, TPB, 0, b->stream>>>(c, b->prim, b->met, b->red + 4);
   CBNS_HIP(b, hipGetLastError());
-  k_dt_final<<<1, TPB, 0, b->stream>>>(b->red + 4, nb, c.CFL, b->red);
+  k_dt_final<<<1, TPB, 0, b->stream>>>(b->red + 4, nb, c.CFL, INFINITY, b->red);
   CBNS_HIP(b, hipGetLastError());
   CBNS_HIP(b, hipMemcpyAsync(b->h_scalar, b->red, 2 * sizeof(double), hipMemcpyDeviceToHost,
                              b->stream));
   CBNS_HIP(b, hipStreamSynchronize(b->stream));
   *dt_host = b->h_scalar[0];
+  return CBNS_OK;
+}
+
+// The time step from per-workgroup maxima another pass wrote (the viscous
+// flux pass of a step's first stage, k_viscous.hip): dt, limited to cap,
+// into out[0] on the device, where the stage kernels read it.
+int launch_dt_from_partials(cbns_block* b, const double* partial, int n, double cap,
+                            double* out) {
+  k_dt_final<<<1, TPB, 0, b->stream>>>(partial, n, b->c.CFL, cap, out);
+  CBNS_HIP(b, hipGetLastError());
   return CBNS_OK;
 }
 

@@ -104,6 +104,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE_X,
 
 struct StageArgs {
   double dt;
+  const double* dtp;    // dt on the device (the folded time step) or null: use dt
   int stage;
   double* prim;         // primitives of the stage output (AX == 2)
   const double* Qn;     // level-n state (stages 2, 3)
@@ -121,6 +122,10 @@ template <int AX, class F, bool HYB>
 __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves_per_eu(WPE_M, WPE_M))) k_march(Consts c, StageArgs sa,
                                                             const double* __restrict__ met) {
   constexpr bool XI = AX == xi_axis<F>() && FUSE_XI;
+  // the update's time step; a NaN step (non-physical state) leaves the
+  // stage output unwritten, and the caller reports it
+  const double dt = AX == 2 && sa.dtp ? *sa.dtp : sa.dt;
+  if (AX == 2 && !(dt == dt)) return;
   const int ax_t = AX == 1 ? 2 : 1;  // transverse axis other than i
   const int lane = threadIdx.x;
   // XI in the eta march: a wavefront updates 63 nodes, its lane 0 sits one
@@ -263,9 +268,9 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
         df *= inv_j;
         const double qin = q[2][n];
         double out;
-        if (sa.stage == 1) out = qin - sa.dt * df;
-        else if (sa.stage == 2) out = 0.75 * qnv[n] + 0.25 * (qin - sa.dt * df);
-        else out = (qnv[n] + 2.0 * (qin - sa.dt * df)) * (1.0 / 3.0);
+        if (sa.stage == 1) out = qin - dt * df;
+        else if (sa.stage == 2) out = 0.75 * qnv[n] + 0.25 * (qin - dt * df);
+        else out = (qnv[n] + 2.0 * (qin - dt * df)) * (1.0 / 3.0);
         if (own) sa.Qout[n * S + o] = out;
         qn_out[n] = out;
       }
@@ -318,8 +323,8 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
     k_xfaces<F, HYB><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fl, fx);
     CBNS_HIP(b, hipGetLastError());
   }
-  StageArgs sa{dt, stage, b->prim, b->Q, qin, qout, fx, b->rhs, nullptr,
-               HYB ? fl + S : nullptr, HYB ? fl : nullptr};
+  StageArgs sa{dt, b->dt_fold ? b->red + 2 : nullptr, stage, b->prim, b->Q, qin, qout, fx,
+               b->rhs, nullptr, HYB ? fl + S : nullptr, HYB ? fl : nullptr};
   const int ni = upd_hi(c, 0) - upd_lo(c, 0) + 1;
   {
     const int nt = upd_hi(c, 2) - upd_lo(c, 2) + 1;
@@ -332,9 +337,16 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
     CBNS_HIP(b, hipGetLastError());
   }
   if (c.viscous) {
-    int rc = sizeof(F) == sizeof(double) ? launch_viscous(b, b->vtmp, nullptr, b->rhs, 1)
-                                         : launch_viscous_f32(b, b->vtmp, b->rhs);
+    // first stage of a folded step: the flux pass also forms dt
+    double* lam = b->dt_fold && stage == 1 ? b->lam_part : nullptr;
+    int rc = sizeof(F) == sizeof(double) ? launch_viscous(b, b->vtmp, nullptr, b->rhs, 1, lam)
+                                         : launch_viscous_f32(b, b->vtmp, b->rhs, lam);
     if (rc) return rc;
+    if (lam) {
+      ProfScope ps(b, CBNS_K_DT);
+      if ((rc = launch_dt_from_partials(b, lam, (int)visc_flux_blocks(c), b->dt_cap, b->red + 2)))
+        return rc;
+    }
   }
   {
     const int nt = upd_hi(c, 1) - upd_lo(c, 1) + 1;

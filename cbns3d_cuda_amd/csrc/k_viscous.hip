@@ -168,15 +168,25 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
   }
 }
 
-template <class T>
+// LAM: also the spectral-radius maximum of the workgroup's nodes in
+// [0, IM) x [0, JM) x [0, KM) (node_lambda, the time step's per-node term),
+// written to lam_part[workgroup]; NaN if any node's is NaN.
+template <class T, bool LAM>
 __global__ void __launch_bounds__(64 * VROWS) k_visc_fluxes(Consts c, TileMarch tm,
                                                      const double* __restrict__ prim,
                                                      const double* __restrict__ met,
                                                      double* __restrict__ mu,
-                                                     T* __restrict__ vf) {
-  int i, j, k0, k1;
-  bool has, own;
-  if (!tile_march_lane(tm, i, j, k0, k1, has, own)) return;
+                                                     T* __restrict__ vf,
+                                                     double* __restrict__ lam_part) {
+  int i = 0, j = 0, k0 = 0, k1 = 0;
+  bool has = false, own = false;
+  if (!tile_march_lane(tm, i, j, k0, k1, has, own)) {
+    if (!LAM) return;
+    k1 = k0;  // no node; the wavefront still joins the workgroup's reduction
+  }
+  double lam = -INFINITY;
+  int nan_seen = 0;
+  const bool lam_line = LAM && own && i >= 0 && i < c.IM && j >= 0 && j < c.JM;
   const long S = c.S;
   const int sz = c.PXY;
   const double* f0[4] = {prim + P_U * S, prim + P_V * S, prim + P_W * S, prim + P_T * S};
@@ -196,6 +206,19 @@ __global__ void __launch_bounds__(64 * VROWS) k_visc_fluxes(Consts c, TileMarch 
       dx[f] = dxi(c, i, win[f][2], f0[f] + base + k * sz);
     }
     if (own) visc_fluxes_node<T>(c, i, j, k, win, dx, prim, met, mu, vf);
+    if (LAM && lam_line && k >= 0 && k < c.KM) {
+      const double sl = node_lambda(c, prim, met, idx(c, i, j, k));
+      if (isnan(sl)) nan_seen = 1;
+      else lam = fmax(lam, sl);
+    }
+  }
+  if (LAM) {
+    __shared__ double lds[VROWS];
+    const int tid = threadIdx.x + threadIdx.y * BOX_X;
+    lam = block_max(lam, lds, tid, 64 * VROWS);
+    __syncthreads();
+    const int any_nan = __syncthreads_or(nan_seen);
+    if (tid == 0) lam_part[blockIdx.x] = any_nan ? NAN : lam;
   }
 }
 
@@ -253,21 +276,35 @@ __global__ void __launch_bounds__(64 * VROWS) k_visc_div(Consts c, TileMarch tm,
 
 }  // namespace
 
-// vf: 12*S scratch (of T); mu may be null.  mode 0 writes dfv (4*S) to
-// `out`, mode 1 subtracts the divergence from the right-hand side (5*S).
-template <class T>
-int viscous(cbns_block* b, T* vf, double* mu, double* out, int mode) {
-  const Consts& c = b->c;
+// Workgroups of the viscous flux pass (the entries lam_part needs).
+static TileMarch visc_flux_tiles(const Consts& c) {
   Box vb;
   for (int a = 0; a < 3; a++) {
     vb.lo[a] = -2;
     vb.n[a] = axis_n(c, a) + 4;
   }
+  return tile_march(vb, VISC_CHUNK, VROWS, 2);
+}
+
+long visc_flux_blocks(const Consts& c) { return (long)tile_march_blocks(visc_flux_tiles(c)); }
+
+// vf: 12*S scratch (of T); mu may be null.  mode 0 writes dfv (4*S) to
+// `out`, mode 1 subtracts the divergence from the right-hand side (5*S).
+// lam_part (visc_flux_blocks entries) or null: the flux pass also forms the
+// time step's per-workgroup spectral-radius maxima.
+template <class T>
+int viscous(cbns_block* b, T* vf, double* mu, double* out, int mode, double* lam_part) {
+  const Consts& c = b->c;
   {
     ProfScope ps(b, CBNS_K_VISC_FLUX);
-    const TileMarch tm = tile_march(vb, VISC_CHUNK, VROWS, 2);
-    k_visc_fluxes<T><<<tile_march_blocks(tm), dim3(BOX_X, VROWS), 0, b->stream>>>(c, tm, b->prim,
-                                                                          b->met, mu, vf);
+    const TileMarch tm = visc_flux_tiles(c);
+    const dim3 grid(tile_march_blocks(tm)), block(BOX_X, VROWS);
+    if (lam_part)
+      k_visc_fluxes<T, true><<<grid, block, 0, b->stream>>>(c, tm, b->prim, b->met, mu, vf,
+                                                            lam_part);
+    else
+      k_visc_fluxes<T, false><<<grid, block, 0, b->stream>>>(c, tm, b->prim, b->met, mu, vf,
+                                                             nullptr);
     CBNS_HIP(b, hipGetLastError());
   }
   Box ub;
@@ -282,13 +319,14 @@ int viscous(cbns_block* b, T* vf, double* mu, double* out, int mode) {
   return CBNS_OK;
 }
 
-int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode) {
-  return viscous<double>(b, vf, mu, out, mode);
+int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode,
+                   double* lam_part) {
+  return viscous<double>(b, vf, mu, out, mode, lam_part);
 }
 
 // precision form: the same scratch holds 12*S floats
-int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs) {
-  return viscous<float>(b, reinterpret_cast<float*>(scratch), nullptr, rhs, 1);
+int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs, double* lam_part) {
+  return viscous<float>(b, reinterpret_cast<float*>(scratch), nullptr, rhs, 1, lam_part);
 }
 
 }  // namespace cbns

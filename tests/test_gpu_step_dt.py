@@ -1,0 +1,75 @@
+"""The time step of cbns_rk3_step on viscous blocks.
+
+There the first stage's viscous flux pass forms the step's dt on the device
+(the per-node spectral-radius sum of src/time_step_kernel.cu:43-94 and its
+maximum), instead of a separate cbns_calc_dt pass.  It must equal
+cbns_calc_dt bit for bit, including the limit to the rest of the run
+(src/Block3d.cpp:241-246), and a NaN time step must be reported with the
+state left as it was (the reference throws before the update,
+src/calc_dt.cu:56-58).
+"""
+import numpy as np
+import pytest
+
+from cbns3d_cuda_amd import cases
+from cbns3d_cuda_amd.block import BUF_Q, F_PRIM, F_Q, FORM_FP32, Block, CbnsError
+
+pytestmark = pytest.mark.gpu
+
+CASES = {
+    "tgv": lambda: cases.taylor_green(16),
+    "cgrid_viscous": lambda: cases.c_grid_channel(viscous=1),
+    "box_70x9x8_rpp_visc": lambda: cases.smooth_box(
+        70, 9, 8, viscous=1, bc=(cases.BC_REF, cases.BC_PERIODIC, cases.BC_PERIODIC)),
+}
+
+
+def _block(case, form=0):
+    b = Block(case.params)
+    b.calc_metrics(case.x, case.y, case.z, case.period)
+    b.set_form(form)
+    b.set_conservative(cases.q_ref_aos(case))
+    b.set_bc_calc_primitive(BUF_Q)
+    return b
+
+
+@pytest.mark.parametrize("form", [0, FORM_FP32])
+@pytest.mark.parametrize("name", list(CASES))
+def test_step_dt_equals_calc_dt(name, form):
+    case = CASES[name]()
+    print("case", case.name, "form", form, flush=True)
+    b = _block(case, form)
+    try:
+        t = 0.0
+        for _ in range(3):
+            dt_ref = b.calc_dt()
+            t_new, dt = b.rk3_step(1e9, t)
+            assert dt == dt_ref  # bit for bit
+            assert t_new == t + dt_ref
+            t = t_new
+        # limited to the rest of the run
+        dt_ref = b.calc_dt()
+        t_end = t + 0.25 * dt_ref
+        t_new, dt = b.rk3_step(t_end, t)
+        assert dt == t_end - t
+        assert t_new == t + (t_end - t)
+    finally:
+        b.free()
+
+
+def test_nan_step_reported_and_state_kept():
+    case = cases.taylor_green(16)
+    print("case", case.name, flush=True)
+    b = _block(case)
+    try:
+        t, _ = b.rk3_step(1e9, 0.0)
+        q0 = b.get_field(F_Q, 5)
+        prim = b.get_field(F_PRIM, 6)
+        prim[4][tuple(s // 2 for s in prim[4].shape)] = np.nan  # one node's pressure
+        b.set_field(F_PRIM, prim)
+        with pytest.raises(CbnsError) as e:
+            b.rk3_step(1e9, t)
+        assert e.value.code == 3  # CBNS_ERR_NAN
+        assert np.array_equal(b.get_field(F_Q, 5), q0)
+    finally:
+        b.free()
