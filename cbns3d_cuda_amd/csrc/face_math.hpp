@@ -53,30 +53,6 @@ struct GasT {
   T gamma, gam1, gam1_inv;
 };
 
-// WENO5-JS at the right face of the centre point c, src/weno.cu:13-44.
-// Weights 0.1/IS1^2 : 0.6/IS2^2 : 0.3/IS3^2 are scaled by IS1^2 IS2^2 IS3^2.
-template <class T>
-__device__ __forceinline__ T weno5(T a, T b, T c, T d, T e) {
-  const T eps = T(1.0e-6);
-  const T d1 = a - T(2) * b + c;
-  const T d2 = a - T(4) * b + T(3) * c;
-  const T d3 = b - T(2) * c + d;
-  const T d4 = b - d;
-  const T d5 = c - T(2) * d + e;
-  const T d6 = T(3) * c - T(4) * d + e;
-  const T IS1 = T(13.0 / 12.0) * d1 * d1 + T(0.25) * d2 * d2 + eps;
-  const T IS2 = T(13.0 / 12.0) * d3 * d3 + T(0.25) * d4 * d4 + eps;
-  const T IS3 = T(13.0 / 12.0) * d5 * d5 + T(0.25) * d6 * d6 + eps;
-  const T s1 = IS1 * IS1, s2 = IS2 * IS2, s3 = IS3 * IS3;
-  const T w1 = T(0.1) * (s2 * s3);
-  const T w2 = T(0.6) * (s1 * s3);
-  const T w3 = T(0.3) * (s1 * s2);
-  const T f1 = T(2) * a - T(7) * b + T(11) * c;
-  const T f2 = -b + T(5) * c + T(2) * d;
-  const T f3 = T(2) * c + T(5) * d - e;
-  return (w1 * f1 + w2 * f2 + w3 * f3) * fast_rcp(T(6) * (w1 + w2 + w3));
-}
-
 // Roe-averaged characteristic basis of a face, src/calc_eigenvectors_roe.cu:28-150.
 template <class T>
 struct Basis {
@@ -162,20 +138,6 @@ __device__ __forceinline__ void roe_basis(const GasT<T>& g, T rL, T uL, T vL, T 
   const T HL = g.gamma * pL * irL * g.gam1_inv + T(0.5) * (uL * uL + vL * vL + wL * wL);
   const T HR = g.gamma * pR * irR * g.gam1_inv + T(0.5) * (uR * uR + vR * vR + wR * wR);
   roe_basis_h(g, rL, uL, vL, wL, HL, rR, uR, vR, wR, HR, n_x, n_y, n_z, B);
-}
-
-// W = L q for one node (q = rho, rho u, rho v, rho w, E).
-// Rows 0 and 2 are A +/- Bn, row 1 is rho - 2A with
-//   A = K (E - u.m + q2/2 rho),  Bn = o (qn rho - n.m).
-template <class T>
-__device__ __forceinline__ void char_project(const Basis<T>& B, const T q[5], T W[5]) {
-  const T A = B.K * (q[4] - (B.uu * q[1] + B.vv * q[2] + B.ww * q[3]) + T(0.5) * B.q2 * q[0]);
-  const T Bn = B.o * (B.qn * q[0] - (B.nx * q[1] + B.ny * q[2] + B.nz * q[3]));
-  W[0] = A + Bn;
-  W[1] = q[0] - T(2) * A;
-  W[2] = A - Bn;
-  W[3] = -B.ql * q[0] + B.lx * q[1] + B.ly * q[2] + B.lz * q[3];
-  W[4] = -B.qm * q[0] + B.mx * q[1] + B.my * q[2] + B.mz * q[3];
 }
 
 // A reconstructed face state: primitives plus 1/rho, |u|^2/2 and the
