@@ -454,9 +454,11 @@ int cbns_rk3_step(cbns_block* b, double t_end, double* t_cur, double* dt_out) {
     // the step.  A NaN step leaves the state untouched and is reported.
     if ((rc = ensure_stage_scratch(b))) return rc;
     b->dt_fold = true;
+    b->step_internal = true;
     b->dt_cap = t_end - *t_cur;
     for (int s = 1; s <= 3 && !rc; s++) rc = cbns_rk3_stage(b, 0.0, s);
     b->dt_fold = false;
+    b->step_internal = false;
     if (rc) return rc;
     CBNS_HIP(b, hipMemcpyAsync(b->h_scalar + 2, b->red + 2, 2 * sizeof(double),
                                hipMemcpyDeviceToHost, b->stream));
@@ -471,9 +473,10 @@ int cbns_rk3_step(cbns_block* b, double t_end, double* t_cur, double* dt_out) {
   if (t_end - *t_cur < dt) dt = t_end - *t_cur;
   *t_cur += dt;
   if (dt_out) *dt_out = dt;
-  for (int s = 1; s <= 3; s++)
-    if ((rc = cbns_rk3_stage(b, dt, s))) return rc;
-  return CBNS_OK;
+  b->step_internal = true;
+  for (int s = 1; s <= 3 && !rc; s++) rc = cbns_rk3_stage(b, dt, s);
+  b->step_internal = false;
+  return rc;
 }
 
 int cbns_set_form(cbns_block* b, int32_t form) {

@@ -249,6 +249,8 @@ struct cbns_block {
   // cbns_rk3_step of a viscous block: stage 1's viscous flux pass also forms
   // the step's dt (limited to dt_cap) on the device, where the stages read it
   bool dt_fold = false;
+  // inside cbns_rk3_step: stages 1-2 skip the primitives nothing reads
+  bool step_internal = false;
   double dt_cap = 0.0;
   int red_n = 0;
   double* h_scalar = nullptr;  // pinned host scalars [4]

@@ -115,6 +115,7 @@ struct StageArgs {
   const double* dfv;    // viscous divergence or nullptr
   const uint8_t* flags; // troubled faces of the march axis (HYB)
   const uint8_t* flags_x;  // troubled xi-faces (HYB, fused xi-faces)
+  int pmask;            // AX == 2: primitives to write (bit n: field n of rho, u, v, w, p, T)
 };
 
 // AX = 1: march along eta, rhs = dE + dF.  AX = 2: march along zeta, finish.
@@ -278,12 +279,12 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
       if (own) {
         double rr, uu, vv, ww, pp;
         q_to_prim(g, qn_out, rr, uu, vv, ww, pp);
-        sa.prim[o] = rr;
-        sa.prim[S + o] = uu;
-        sa.prim[2 * S + o] = vv;
-        sa.prim[3 * S + o] = ww;
-        sa.prim[4 * S + o] = pp;
-        if (c.viscous) sa.prim[5 * S + o] = c.gM2 * pp * fast_rcp(rr);
+        if (sa.pmask & 1) sa.prim[o] = rr;
+        if (sa.pmask & 2) sa.prim[S + o] = uu;
+        if (sa.pmask & 4) sa.prim[2 * S + o] = vv;
+        if (sa.pmask & 8) sa.prim[3 * S + o] = ww;
+        if (sa.pmask & 16) sa.prim[4 * S + o] = pp;
+        if (sa.pmask & 32) sa.prim[5 * S + o] = c.gM2 * pp * fast_rcp(rr);
       }
     }
 #pragma unroll
@@ -323,8 +324,14 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
     k_xfaces<F, HYB><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fl, fx);
     CBNS_HIP(b, hipGetLastError());
   }
+  // Primitives the stage output needs: all of them, except inside
+  // cbns_rk3_step, where the first two stages' outputs feed only the next
+  // stage (the marches derive what they need from Q; the viscous flux pass
+  // reads u, v, w, T) and the third stage writes all of them.
+  const int pall = c.viscous ? 0x3F : 0x1F;
+  const int pmask = b->step_internal && stage < 3 ? (c.viscous ? 0x2E : 0) : pall;
   StageArgs sa{dt, b->dt_fold ? b->red + 2 : nullptr, stage, b->prim, b->Q, qin, qout, fx,
-               b->rhs, nullptr, HYB ? fl + S : nullptr, HYB ? fl : nullptr};
+               b->rhs, nullptr, HYB ? fl + S : nullptr, HYB ? fl : nullptr, pmask};
   const int ni = upd_hi(c, 0) - upd_lo(c, 0) + 1;
   {
     const int nt = upd_hi(c, 2) - upd_lo(c, 2) + 1;

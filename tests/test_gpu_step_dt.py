@@ -1,4 +1,5 @@
-"""The time step of cbns_rk3_step on viscous blocks.
+"""The time step of cbns_rk3_step on viscous blocks, and the primitives a
+step leaves behind.
 
 There the first stage's viscous flux pass forms the step's dt on the device
 (the per-node spectral-radius sum of src/time_step_kernel.cu:43-94 and its
@@ -71,5 +72,33 @@ def test_nan_step_reported_and_state_kept():
             b.rk3_step(1e9, t)
         assert e.value.code == 3  # CBNS_ERR_NAN
         assert np.array_equal(b.get_field(F_Q, 5), q0)
+    finally:
+        b.free()
+
+
+@pytest.mark.parametrize("name", ["tgv", "cgrid_viscous", "box_70x9x8_rpp_visc", "vortex",
+                                  "cgrid_inviscid"])
+def test_step_leaves_primitives_of_its_state(name):
+    """cbns_rk3_step's first two stages write only the primitives the next
+    stage reads; after the step every primitive is that of the new state
+    (as set_bc_calc_primitive derives them, to rounding: the stage forms
+    1/rho by Newton steps, the boundary pass by division)."""
+    case = {**CASES, "vortex": lambda: cases.isentropic_vortex(20),
+            "cgrid_inviscid": lambda: cases.c_grid_channel(viscous=0)}[name]()
+    print("case", case.name, flush=True)
+    b = _block(case)
+    try:
+        t = 0.0
+        for _ in range(2):
+            t, _ = b.rk3_step(1e9, t)
+        n = 6 if case.params.viscous else 5
+        p_step = b.get_field(F_PRIM, n)
+        b.set_bc_calc_primitive(BUF_Q)
+        p_ref = b.get_field(F_PRIM, n)
+        for f in range(n):
+            scale = np.abs(p_ref[f]).max()
+            err = np.abs(p_step[f] - p_ref[f]).max() / scale
+            print("field", f, "max relative difference", err, flush=True)
+            assert err < 1e-14
     finally:
         b.free()
