@@ -63,17 +63,20 @@ def kernel_work(name, p):
     """(kind, units of work per launch) for the per-kernel roofline."""
     faces, upd = face_counts(p)
     nodes = upd[0] * upd[1] * upd[2]
-    if name == "march_eta":  # xi- and eta-faces (xi fused into the eta march)
-        return "flop", (faces[0] + faces[1]) * FLOPS_PER_FACE
+    if name == "march_eta":  # eta-faces
+        return "flop", faces[1] * FLOPS_PER_FACE
     if name == "xfaces":
         return "flop", faces[0] * FLOPS_PER_FACE
-    if name == "march_zeta":
-        return "flop", faces[2] * FLOPS_PER_FACE
+    if name == "march_zeta":  # xi- and zeta-faces (xi fused into the zeta march)
+        return "flop", (faces[0] + faces[2]) * FLOPS_PER_FACE
     if name == "visc_flux":
         return "byte", (p.IM + 4) * (p.JM + 4) * (p.KM + 4) * VISC_FLUX_BYTES_PER_NODE
     if name == "visc_div":
         return "byte", nodes * VISC_DIV_BYTES_PER_NODE
     if name == "dt":
+        if p.viscous:  # the final pass over the viscous flux pass's maxima
+            n = [(p.IM + 4 + 59) // 60, (p.JM + 4 + 3) // 4, (p.KM + 4 + 7) // 8]
+            return "byte", n[0] * n[1] * n[2] * 8
         return "byte", p.IM * p.JM * p.KM * DT_BYTES_PER_NODE
     if name == "indicator":  # per axis launch: 6-node stencil + 2 node states read once, 1 B out
         return "byte", sum(faces) // 3 * (5 + 8) * 8

@@ -3,14 +3,15 @@
 //   k_march<1>   lanes along i, each thread marches a chunk of nodes along
 //                eta holding the 6-node stencil and the previous face flux in
 //                registers: each node is loaded once per sweep and each face
-//                is computed once.  One of the two marches (xi_axis) also
-//                computes, at every node, the xi-face on its right; the left
-//                one comes from the lane below by a wavefront shuffle.  In
-//                the eta march (FP64) a wavefront updates 63 nodes, its lane
-//                0 only supplies that face, and the xi-stencils are loaded;
-//                in the zeta march (float faces) they come from the
+//                is computed once.  One of the two marches (xi_axis: the
+//                zeta march) also computes, at every node, the xi-face on its
+//                right; the left one comes from the lane below by a wavefront
+//                shuffle.  In the zeta march the xi-stencils come from the
 //                neighbouring lanes' windows and a wavefront updates 58
-//                nodes (march_width).  Writes rhs = dE + dF (or dF).
+//                nodes; in the eta march (the other placement) a wavefront
+//                updates 63 nodes, its lane 0 only supplies that face, and the
+//                xi-stencils are loaded (march_width).  Writes rhs = dF, or
+//                dE + dF with the xi-faces here.
 //                (FUSE_XI=0: a separate k_xfaces pass writes the xi-face
 //                fluxes to HBM and the march reads them back.)
 //   [viscous]    k_visc_fluxes + k_visc_div subtract the viscous divergence
@@ -46,13 +47,13 @@ constexpr int CHUNK = MARCH_CHUNK;  // nodes per thread along the march axis
 #ifndef FUSE_XI
 #define FUSE_XI 1  // xi-faces inside a march (no xi-face pass through HBM)
 #endif
-// The march that also forms the xi-faces.  FP64 (F = double): the eta march
-// (in the zeta march, which also carries the RK update, it would need more
-// than 256 VGPRs and spill).  FP32 faces: the zeta march, whose update
-// traffic then overlaps the extra face work (measured: +9 % for the FP32
-// form, -6 % for FP64).
+// The march that also forms the xi-faces: the zeta march, whose RK-update
+// traffic then overlaps the extra face work (the eta march alone is bound
+// by its face arithmetic, the zeta march by its update traffic).  Measured
+// against the eta march: +3 to +5 % of the FP64 step, +9 % of the FP32 step.
+// Both placements are supported (xi_axis 1 or 2).
 template <class F>
-constexpr int xi_axis() { return sizeof(F) == sizeof(double) ? 1 : 2; }
+constexpr int xi_axis() { return 2; }
 // nodes a march wavefront updates along i (see k_march)
 template <int AX, class F>
 constexpr int march_width() {
@@ -182,11 +183,14 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
       for (int n = 0; n < 5; n++) q[s][n] = q[s + 1][n];
     o = base + m * st;
     load_q(Q, S, o + 3 * st, q[5]);
-    // the zeta sweep's update operands are fetched before the face work so
-    // their latency overlaps it
     const double jo = jb;  // J at node m, loaded as the previous face's right node
+    // the zeta sweep's update operands are fetched before the face work so
+    // their latency overlaps it; with FP64 faces and the xi-faces in this
+    // march they are fetched where they are used (hoisted they would push
+    // the kernel past 256 VGPRs)
+    constexpr bool HOIST = AX == 2 && !(XI && sizeof(F) == sizeof(double));
     double rin[5], qnv[5];
-    if (AX == 2) {
+    if (HOIST) {
 #pragma unroll
       for (int n = 0; n < 5; n++) {
         rin[n] = sa.rhs[n * S + o];
@@ -259,6 +263,13 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
         sa.rhs[n * S + o] = (fxn[o] - fxn[o - 1]) + (f[n] - fprev[n]);
       }
     } else {
+      if (!HOIST) {
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          rin[n] = sa.rhs[n * S + o];
+          qnv[n] = sa.stage > 1 ? sa.Qn[n * S + o] : 0.0;
+        }
+      }
       const double inv_j = 1.0 / jo;
       double qn_out[5];
 #pragma unroll
