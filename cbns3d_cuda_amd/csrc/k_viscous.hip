@@ -19,7 +19,9 @@
 // four fields do not fit a 4 MB XCD L2 and would be re-read from HBM).
 // In the flux pass a wavefront holds its 60-node tile along i and two nodes
 // on either side; the xi-derivatives take their neighbours from the adjacent
-// lanes by wavefront shuffles (LDS crossbar) instead of 16 loads per node.
+// lanes by wavefront shuffles (LDS crossbar) instead of 16 loads per node,
+// and the eta-derivatives from a plane buffer in LDS that holds the
+// workgroup's rows and two rows on either side.
 // The flux pass waits on the vector-memory pipeline (address and data units
 // ~90 % busy at 256^3), not on HBM bytes: remapping its tiles so that
 // neighbouring rows share an XCD's L2 fetched 28 % fewer bytes and ran no
@@ -38,6 +40,13 @@ namespace {
 #define VISC_ROWS 4  // workgroup rows along j (64 x VISC_ROWS threads)
 #endif
 constexpr int VROWS = VISC_ROWS;
+#ifndef VISC_FLUX_ROWS
+#define VISC_FLUX_ROWS 4  // flux-pass workgroup rows along j
+#endif
+#ifndef VISC_FLUX_CHUNK
+#define VISC_FLUX_CHUNK 8  // flux-pass nodes along k per thread
+#endif
+constexpr int FROWS = VISC_FLUX_ROWS;
 
 // 4th-order derivative along axis a at node m of the viscous range:
 // one-sided at the two outermost layers of a reference axis
@@ -76,16 +85,16 @@ __device__ __forceinline__ void xi_neighbours(V v, V n[4]) {
 
 // d/dxi of v at node i from the neighbouring lanes (every lane takes part);
 // the one-sided layers of a reference axis read memory (dgrad, from f at the
-// node)
+// node) where `node` says the lane's row is in the array
 __device__ __forceinline__ double dxi(const Consts& c, int i, double v,
-                                      const double* __restrict__ f) {
+                                      const double* __restrict__ f, bool node) {
   double nb[4];
   xi_neighbours(v, nb);
   const double d =
       (1.0 / 12.0) * nb[0] - 2.0 / 3.0 * nb[1] + (2.0 / 3.0) * nb[2] - 1.0 / 12.0 * nb[3];
   // (only the four outer nodes -2, -1, IM, IM+1 of the viscous range: the
   // halo lanes beyond them hold no node of the range)
-  if (!periodic(c, 0) && (i == -2 || i == -1 || i == c.IM || i == c.IM + 1))
+  if (node && !periodic(c, 0) && (i == -2 || i == -1 || i == c.IM || i == c.IM + 1))
     return dgrad(c, 0, i, f, 1);
   return d;
 }
@@ -103,7 +112,7 @@ __device__ __forceinline__ double dwin(const double w[5]) {
 template <class T>
 __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, int k,
                                                  const double win[4][5],
-                                                 const double dx[4],
+                                                 const double dx[4], const double de[4],
                                                  const double* __restrict__ prim,
                                                  const double* __restrict__ met,
                                                  double* __restrict__ mu,
@@ -114,7 +123,7 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
   const double* V = prim + P_V * S + o;
   const double* W = prim + P_W * S + o;
   const double* TT = prim + P_T * S + o;
-  const int sy = c.PX, sz = c.PXY;
+  const int sz = c.PXY;
   // zeta: the register window, except at the one-sided outer layers of a
   // reference axis
   const bool zc = periodic(c, 2) || (k >= 0 && k <= c.KM - 1);
@@ -122,10 +131,10 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
   const double vz = zc ? dwin(win[1]) : dgrad(c, 2, k, V, sz);
   const double wz = zc ? dwin(win[2]) : dgrad(c, 2, k, W, sz);
   const double tz = zc ? dwin(win[3]) : dgrad(c, 2, k, TT, sz);
-  T u_xi_l = (T)dx[0], u_eta_l = (T)dgrad(c, 1, j, U, sy), u_zeta_l = (T)uz;
-  const T v_xi_l = (T)dx[1], v_eta_l = (T)dgrad(c, 1, j, V, sy), v_zeta_l = (T)vz;
-  const T w_xi_l = (T)dx[2], w_eta_l = (T)dgrad(c, 1, j, W, sy), w_zeta_l = (T)wz;
-  const T T_xi = (T)dx[3], T_eta = (T)dgrad(c, 1, j, TT, sy), T_zeta = (T)tz;
+  T u_xi_l = (T)dx[0], u_eta_l = (T)de[0], u_zeta_l = (T)uz;
+  const T v_xi_l = (T)dx[1], v_eta_l = (T)de[1], v_zeta_l = (T)vz;
+  const T w_xi_l = (T)dx[2], w_eta_l = (T)de[2], w_zeta_l = (T)wz;
+  const T T_xi = (T)dx[3], T_eta = (T)de[3], T_zeta = (T)tz;
 
   // src/viscous_terms_kernel.cu:59-105
   const double T_l = win[3][2];
@@ -168,44 +177,97 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
   }
 }
 
+// The workgroup's rows (64 x FROWS lanes, one row of j per wavefront) and
+// the two rows on either side along j: at each plane k the lanes put their
+// u,v,w,T into a plane buffer in LDS (the halo rows loaded by the first four
+// wavefronts), and the eta-derivatives are differenced from it instead of
+// from four loads per field.  Two buffers alternate with k, so one barrier
+// per plane orders the writes of plane k+1 after the reads of plane k-1.
+constexpr int VPLANE_ROWS = FROWS + 4;
+static_assert(FROWS >= 4, "the halo rows are loaded by the first four wavefronts");
+
+// Where a workgroup's lane of the flux pass sits: like tile_march_lane, but
+// rows past the box along j stay (they load the values the last rows'
+// stencils read and take part in the barriers).  false for the whole
+// workgroup past the tile count.
+__device__ inline bool visc_flux_lane(const TileMarch& t, int& i, int& j, int& k0, int& k1,
+                                      bool& has, bool& own, bool& row_in) {
+  const unsigned nb = (unsigned)t.ntx * t.nty * t.nkc;
+  const unsigned per = (nb + N_XCD - 1) / N_XCD;
+  const unsigned lb = (blockIdx.x % N_XCD) * per + blockIdx.x / N_XCD;
+  if (lb >= nb) return false;
+  const unsigned tx = lb % t.ntx, r = lb / t.ntx;
+  const unsigned ty = r % t.nty, kc = r / t.nty;
+  const int lane = (int)threadIdx.x;
+  const int ti = (int)tx * (BOX_X - 2 * t.xh) - t.xh + lane, tj = ty * t.rows + threadIdx.y;
+  row_in = tj < t.box.n[1];
+  has = ti < t.box.n[0] + t.xh;
+  own = row_in && lane >= t.xh && lane < BOX_X - t.xh && ti < t.box.n[0];
+  i = t.box.lo[0] + ti;
+  j = t.box.lo[1] + tj;
+  k0 = t.box.lo[2] + (int)kc * t.chunk;
+  k1 = min(k0 + t.chunk, t.box.lo[2] + t.box.n[2]);
+  return true;
+}
+
 // LAM: also the spectral-radius maximum of the workgroup's nodes in
 // [0, IM) x [0, JM) x [0, KM) (node_lambda, the time step's per-node term),
 // written to lam_part[workgroup]; NaN if any node's is NaN.
 template <class T, bool LAM>
-__global__ void __launch_bounds__(64 * VROWS) k_visc_fluxes(Consts c, TileMarch tm,
+__global__ void __launch_bounds__(64 * FROWS) k_visc_fluxes(Consts c, TileMarch tm,
                                                      const double* __restrict__ prim,
                                                      const double* __restrict__ met,
                                                      double* __restrict__ mu,
                                                      T* __restrict__ vf,
                                                      double* __restrict__ lam_part) {
+  __shared__ double plane[2][4][VPLANE_ROWS][BOX_X];
   int i = 0, j = 0, k0 = 0, k1 = 0;
-  bool has = false, own = false;
-  if (!tile_march_lane(tm, i, j, k0, k1, has, own)) {
-    if (!LAM) return;
-    k1 = k0;  // no node; the wavefront still joins the workgroup's reduction
-  }
+  bool has = false, own = false, row_in = false;
+  if (!visc_flux_lane(tm, i, j, k0, k1, has, own, row_in)) return;
+  const int lane = threadIdx.x, ty = threadIdx.y;
+  // the lane's row is loaded if it lies in the array (j >= -2 > -G always)
+  const bool has_row = has && j < c.JM + G;
+  // halo row of this wavefront (the first four): j0-2, j0-1, j0+FROWS, j0+FROWS+1
+  const int hr = ty < 2 ? ty : ty + FROWS;
+  const int jh = j - ty - 2 + hr;
+  const bool halo = ty < 4, has_halo = halo && has && jh < c.JM + G;
+  // eta-stencil from the plane buffer, except at the one-sided outer layers
+  // of a reference axis
+  const bool ec = periodic(c, 1) || (j >= 0 && j <= c.JM - 1);
   double lam = -INFINITY;
   int nan_seen = 0;
   const bool lam_line = LAM && own && i >= 0 && i < c.IM && j >= 0 && j < c.JM;
   const long S = c.S;
   const int sz = c.PXY;
   const double* f0[4] = {prim + P_U * S, prim + P_V * S, prim + P_W * S, prim + P_T * S};
-  const int base = idx(c, i, j, 0);
+  const int base = idx(c, i, j, 0), hbase = idx(c, i, jh, 0);
   double win[4][5];
 #pragma unroll
   for (int f = 0; f < 4; f++)
 #pragma unroll
-    for (int s = 0; s < 4; s++) win[f][s + 1] = has ? f0[f][base + (k0 - 2 + s) * sz] : 0.0;
+    for (int s = 0; s < 4; s++) win[f][s + 1] = has_row ? f0[f][base + (k0 - 2 + s) * sz] : 0.0;
   for (int k = k0; k < k1; k++) {
-    double dx[4];
+    double dx[4], de[4], hv[4];
+    const int pb = k & 1;
+#pragma unroll
+    for (int f = 0; f < 4; f++) hv[f] = has_halo ? f0[f][hbase + k * sz] : 0.0;
 #pragma unroll
     for (int f = 0; f < 4; f++) {
 #pragma unroll
       for (int s = 0; s < 4; s++) win[f][s] = win[f][s + 1];
-      win[f][4] = has ? f0[f][base + (k + 2) * sz] : 0.0;
-      dx[f] = dxi(c, i, win[f][2], f0[f] + base + k * sz);
+      win[f][4] = has_row ? f0[f][base + (k + 2) * sz] : 0.0;
+      dx[f] = dxi(c, i, win[f][2], f0[f] + base + k * sz, has_row);
+      plane[pb][f][ty + 2][lane] = win[f][2];
+      if (halo) plane[pb][f][hr][lane] = hv[f];
     }
-    if (own) visc_fluxes_node<T>(c, i, j, k, win, dx, prim, met, mu, vf);
+    __syncthreads();
+#pragma unroll
+    for (int f = 0; f < 4; f++)
+      de[f] = ec ? (1.0 / 12.0) * plane[pb][f][ty][lane] - 2.0 / 3.0 * plane[pb][f][ty + 1][lane] +
+                       (2.0 / 3.0) * plane[pb][f][ty + 3][lane] -
+                       1.0 / 12.0 * plane[pb][f][ty + 4][lane]
+                 : (own ? dgrad(c, 1, j, f0[f] + base + k * sz, c.PX) : 0.0);
+    if (own) visc_fluxes_node<T>(c, i, j, k, win, dx, de, prim, met, mu, vf);
     if (LAM && lam_line && k >= 0 && k < c.KM) {
       const double sl = node_lambda(c, prim, met, idx(c, i, j, k));
       if (isnan(sl)) nan_seen = 1;
@@ -213,9 +275,9 @@ __global__ void __launch_bounds__(64 * VROWS) k_visc_fluxes(Consts c, TileMarch 
     }
   }
   if (LAM) {
-    __shared__ double lds[VROWS];
+    __shared__ double lds[FROWS];
     const int tid = threadIdx.x + threadIdx.y * BOX_X;
-    lam = block_max(lam, lds, tid, 64 * VROWS);
+    lam = block_max(lam, lds, tid, 64 * FROWS);
     __syncthreads();
     const int any_nan = __syncthreads_or(nan_seen);
     if (tid == 0) lam_part[blockIdx.x] = any_nan ? NAN : lam;
@@ -283,7 +345,7 @@ static TileMarch visc_flux_tiles(const Consts& c) {
     vb.lo[a] = -2;
     vb.n[a] = axis_n(c, a) + 4;
   }
-  return tile_march(vb, VISC_CHUNK, VROWS, 2);
+  return tile_march(vb, VISC_FLUX_CHUNK, FROWS, 2);
 }
 
 long visc_flux_blocks(const Consts& c) { return (long)tile_march_blocks(visc_flux_tiles(c)); }
@@ -298,7 +360,7 @@ int viscous(cbns_block* b, T* vf, double* mu, double* out, int mode, double* lam
   {
     ProfScope ps(b, CBNS_K_VISC_FLUX);
     const TileMarch tm = visc_flux_tiles(c);
-    const dim3 grid(tile_march_blocks(tm)), block(BOX_X, VROWS);
+    const dim3 grid(tile_march_blocks(tm)), block(BOX_X, FROWS);
     if (lam_part)
       k_visc_fluxes<T, true><<<grid, block, 0, b->stream>>>(c, tm, b->prim, b->met, mu, vf,
                                                             lam_part);
