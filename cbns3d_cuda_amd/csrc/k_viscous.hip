@@ -48,6 +48,7 @@ constexpr int VROWS = VISC_ROWS;
 #endif
 constexpr int FROWS = VISC_FLUX_ROWS;
 
+
 // 4th-order derivative along axis a at node m of the viscous range:
 // one-sided at the two outermost layers of a reference axis
 // (src/gradient_kernel.cu:162-253), central with wrapped ghosts otherwise.
@@ -108,13 +109,13 @@ __device__ __forceinline__ double dwin(const double w[5]) {
 // T = double (plain form) or float (precision form: the index derivatives
 // are differenced in double and rounded, stresses, heat flux and fluxes are
 // formed and stored in float).  win[f][s]: field f (u,v,w,T) at k-2+s;
-// dx[f]: d/dxi of field f.
+// dx[f], de[f]: d/dxi, d/deta of field f; m: the node's metrics (XI_X..JAC).
 template <class T>
 __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, int k,
                                                  const double win[4][5],
                                                  const double dx[4], const double de[4],
+                                                 const double m[10],
                                                  const double* __restrict__ prim,
-                                                 const double* __restrict__ met,
                                                  double* __restrict__ mu,
                                                  T* __restrict__ vf) {
   const long S = c.S;
@@ -141,11 +142,9 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
   const double C_T_inf = c.C_T_inf;
   const double mu_d = (1.0 + C_T_inf) / (T_l + C_T_inf) * sqrt(T_l * T_l * T_l);
   if (mu) mu[o] = mu_d;
-  const T xi_x_l = (T)met[XI_X * S + o], xi_y_l = (T)met[XI_Y * S + o], xi_z_l = (T)met[XI_Z * S + o];
-  const T eta_x_l = (T)met[ETA_X * S + o], eta_y_l = (T)met[ETA_Y * S + o],
-          eta_z_l = (T)met[ETA_Z * S + o];
-  const T zeta_x_l = (T)met[ZETA_X * S + o], zeta_y_l = (T)met[ZETA_Y * S + o],
-          zeta_z_l = (T)met[ZETA_Z * S + o];
+  const T xi_x_l = (T)m[XI_X], xi_y_l = (T)m[XI_Y], xi_z_l = (T)m[XI_Z];
+  const T eta_x_l = (T)m[ETA_X], eta_y_l = (T)m[ETA_Y], eta_z_l = (T)m[ETA_Z];
+  const T zeta_x_l = (T)m[ZETA_X], zeta_y_l = (T)m[ZETA_Y], zeta_z_l = (T)m[ZETA_Z];
   T mu_l = (T)(mu_d * c.Re_inv);
   const T txx = T(2.0/3.0)*mu_l*(2*eta_x_l*u_eta_l - eta_y_l*v_eta_l - eta_z_l*w_eta_l + 2*u_xi_l*xi_x_l + 2*u_zeta_l*zeta_x_l - v_xi_l*xi_y_l - v_zeta_l*zeta_y_l - w_xi_l*xi_z_l - w_zeta_l*zeta_z_l);
   const T txy = mu_l*(eta_x_l*v_eta_l + eta_y_l*u_eta_l + u_xi_l*xi_y_l + u_zeta_l*zeta_y_l + v_xi_l*xi_x_l + v_zeta_l*zeta_x_l);
@@ -159,16 +158,16 @@ __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, 
   const T qz = mu_l * (xi_z_l * T_xi + eta_z_l * T_eta + zeta_z_l * T_zeta);
 
   // src/viscous_flux_kernel.cu:57-110
-  const double Jac_a = met[JAC * S + o];
+  const double Jac_a = m[JAC];
   const T uu = (T)win[0][2], vv = (T)win[1][2], ww = (T)win[2][2];
   const T ex = uu * txx + vv * txy + ww * txz - qx;
   const T ey = uu * txy + vv * tyy + ww * tyz - qy;
   const T ez = uu * txz + vv * tyz + ww * tzz - qz;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const T n_x = (T)(met[(3 * d + 0) * S + o] * Jac_a);
-    const T n_y = (T)(met[(3 * d + 1) * S + o] * Jac_a);
-    const T n_z = (T)(met[(3 * d + 2) * S + o] * Jac_a);
+    const T n_x = (T)(m[3 * d + 0] * Jac_a);
+    const T n_y = (T)(m[3 * d + 1] * Jac_a);
+    const T n_z = (T)(m[3 * d + 2] * Jac_a);
     T* E = vf + (long)(4 * d) * S + o;
     E[0] = n_x * txx + n_y * txy + n_z * txz;
     E[S] = n_x * txy + n_y * tyy + n_z * tyz;
@@ -246,11 +245,20 @@ __global__ void __launch_bounds__(64 * FROWS) k_visc_fluxes(Consts c, TileMarch 
   for (int f = 0; f < 4; f++)
 #pragma unroll
     for (int s = 0; s < 4; s++) win[f][s + 1] = has_row ? f0[f][base + (k0 - 2 + s) * sz] : 0.0;
+  // the halo rows are fetched one plane ahead (plane k1 <= KM + 2 lies in
+  // the array), so the barrier does not wait on a load of its own iteration
+  // (the metrics too cost a wavefront per SIMD and ran slower)
+  double hn[4];
+#pragma unroll
+  for (int f = 0; f < 4; f++) hn[f] = has_halo ? f0[f][hbase + k0 * sz] : 0.0;
   for (int k = k0; k < k1; k++) {
     double dx[4], de[4], hv[4];
     const int pb = k & 1;
 #pragma unroll
-    for (int f = 0; f < 4; f++) hv[f] = has_halo ? f0[f][hbase + k * sz] : 0.0;
+    for (int f = 0; f < 4; f++) {
+      hv[f] = hn[f];
+      hn[f] = has_halo ? f0[f][hbase + (k + 1) * sz] : 0.0;
+    }
 #pragma unroll
     for (int f = 0; f < 4; f++) {
 #pragma unroll
@@ -267,7 +275,12 @@ __global__ void __launch_bounds__(64 * FROWS) k_visc_fluxes(Consts c, TileMarch 
                        (2.0 / 3.0) * plane[pb][f][ty + 3][lane] -
                        1.0 / 12.0 * plane[pb][f][ty + 4][lane]
                  : (own ? dgrad(c, 1, j, f0[f] + base + k * sz, c.PX) : 0.0);
-    if (own) visc_fluxes_node<T>(c, i, j, k, win, dx, de, prim, met, mu, vf);
+    if (own) {
+      double m[10];
+#pragma unroll
+      for (int q = 0; q < 10; q++) m[q] = met[q * S + base + k * sz];
+      visc_fluxes_node<T>(c, i, j, k, win, dx, de, m, prim, mu, vf);
+    }
     if (LAM && lam_line && k >= 0 && k < c.KM) {
       const double sl = node_lambda(c, prim, met, idx(c, i, j, k));
       if (isnan(sl)) nan_seen = 1;
