@@ -54,27 +54,23 @@ __host__ __device__ inline int upd_hi(const Consts& c, int a) {
 enum { XI_X = 0, XI_Y, XI_Z, ETA_X, ETA_Y, ETA_Z, ZETA_X, ZETA_Y, ZETA_Z, JAC };
 enum { P_RHO = 0, P_U, P_V, P_W, P_P, P_T };
 
-// Spectral-radius sum of node o, src/time_step_kernel.cu:43-94, one
-// rounding per operation (no contraction) wherever it is inlined, so the
-// time step matches the oracle bit for bit.
-__device__ inline double node_lambda(const Consts& c, const double* __restrict__ prim,
-                                     const double* __restrict__ met, int o) {
+// Spectral-radius sum of a node, src/time_step_kernel.cu:43-94, from its
+// primitives (T_l used only on viscous blocks) and its nine metric terms m
+// (XI_X..ZETA_Z); one rounding per operation (no contraction) wherever it is
+// inlined, so the time step matches the oracle bit for bit.
+__device__ inline double node_lambda_v(const Consts& c, double rr, double uu, double vv, double ww,
+                                       double pp, double T_l, const double* m) {
 #pragma clang fp contract(off)
-  const long S = c.S;
-  const double rr = prim[o], uu = prim[S + o], vv = prim[2 * S + o], ww = prim[3 * S + o];
-  const double pp = prim[4 * S + o];
   const double cs = sqrt(c.gamma * pp / rr);
   double factor_v = 0.0;
   if (c.viscous) {
-    const double T_l = prim[5 * S + o];
     const double mu = (1.0 + c.C_T_inf) / (T_l + c.C_T_inf) * sqrt(T_l * T_l * T_l);
     factor_v = 2.0 * mu * c.C_dt_v * sqrt(uu * uu + vv * vv + ww * ww) / cs / rr;
   }
   double sum = 0.0;
 #pragma unroll
   for (int d = 0; d < 3; d++) {
-    const double n_x = met[(3 * d + 0) * S + o], n_y = met[(3 * d + 1) * S + o],
-                 n_z = met[(3 * d + 2) * S + o];
+    const double n_x = m[3 * d + 0], n_y = m[3 * d + 1], n_z = m[3 * d + 2];
     const double qn = uu * n_x + vv * n_y + ww * n_z;
     const double n_abs = sqrt(n_x * n_x + n_y * n_y + n_z * n_z);
     double tt = fabs(qn) + cs * n_abs;
@@ -82,6 +78,16 @@ __device__ inline double node_lambda(const Consts& c, const double* __restrict__
     sum = d == 0 ? tt : sum + tt;
   }
   return sum;
+}
+// the same, read from node o of the primitive and metric fields
+__device__ inline double node_lambda(const Consts& c, const double* __restrict__ prim,
+                                     const double* __restrict__ met, int o) {
+  const long S = c.S;
+  double m[9];
+#pragma unroll
+  for (int q = 0; q < 9; q++) m[q] = met[q * S + o];
+  return node_lambda_v(c, prim[o], prim[S + o], prim[2 * S + o], prim[3 * S + o], prim[4 * S + o],
+                       c.viscous ? prim[5 * S + o] : 0.0, m);
 }
 
 // Maximum over a wavefront (every lane takes part) and over a workgroup of

@@ -280,11 +280,14 @@ __global__ void __launch_bounds__(64 * FROWS) k_visc_fluxes(Consts c, TileMarch 
 #pragma unroll
       for (int q = 0; q < 10; q++) m[q] = met[q * S + base + k * sz];
       visc_fluxes_node<T>(c, i, j, k, win, dx, de, m, prim, mu, vf);
-    }
-    if (LAM && lam_line && k >= 0 && k < c.KM) {
-      const double sl = node_lambda(c, prim, met, idx(c, i, j, k));
-      if (isnan(sl)) nan_seen = 1;
-      else lam = fmax(lam, sl);
+      if (LAM && lam_line && k >= 0 && k < c.KM) {
+        // u, v, w, T and the metrics from the registers, rho and p loaded
+        const int o = base + k * sz;
+        const double sl = node_lambda_v(c, prim[P_RHO * S + o], win[0][2], win[1][2], win[2][2],
+                                        prim[P_P * S + o], win[3][2], m);
+        if (isnan(sl)) nan_seen = 1;
+        else lam = fmax(lam, sl);
+      }
     }
   }
   if (LAM) {
