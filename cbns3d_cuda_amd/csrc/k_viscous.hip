@@ -34,10 +34,10 @@ namespace cbns {
 namespace {
 
 #ifndef VISC_CHUNK
-#define VISC_CHUNK 8  // nodes along k per thread
+#define VISC_CHUNK 8  // divergence-pass nodes along k per thread
 #endif
 #ifndef VISC_ROWS
-#define VISC_ROWS 4  // workgroup rows along j (64 x VISC_ROWS threads)
+#define VISC_ROWS 8  // divergence-pass workgroup rows along j (64 x VISC_ROWS threads)
 #endif
 constexpr int VROWS = VISC_ROWS;
 #ifndef VISC_FLUX_ROWS
