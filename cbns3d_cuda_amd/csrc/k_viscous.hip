@@ -22,11 +22,12 @@
 // lanes by wavefront shuffles (LDS crossbar) instead of 16 loads per node,
 // and the eta-derivatives from a plane buffer in LDS that holds the
 // workgroup's rows and two rows on either side.
-// The flux pass waits on the vector-memory pipeline (address and data units
-// ~90 % busy at 256^3), not on HBM bytes: remapping its tiles so that
-// neighbouring rows share an XCD's L2 fetched 28 % fewer bytes and ran no
-// faster.  (The divergence pass keeps its loads: there the shuffles cost 30
-// VGPRs, two of its six waves per SIMD and 20 % of its time.)
+// With its stencils loaded, the flux pass waited on the vector-memory
+// pipeline (address and data units ~90 % busy at 256^3) rather than on HBM
+// bytes; the shuffles and the plane buffer cut its loads per node from 46 to
+// 18.  The divergence pass keeps its loads: there the shuffles cost 30
+// VGPRs, two of its six waves per SIMD and 20 % of its time, and a plane
+// buffer for Fv saved nothing (it is bound by its HBM bytes).
 #include "cbns_internal.hpp"
 
 namespace cbns {
@@ -47,7 +48,6 @@ constexpr int VROWS = VISC_ROWS;
 #define VISC_FLUX_CHUNK 8  // flux-pass nodes along k per thread
 #endif
 constexpr int FROWS = VISC_FLUX_ROWS;
-
 
 // 4th-order derivative along axis a at node m of the viscous range:
 // one-sided at the two outermost layers of a reference axis
