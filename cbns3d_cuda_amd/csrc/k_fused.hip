@@ -270,7 +270,7 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
           qnv[n] = sa.stage > 1 ? sa.Qn[n * S + o] : 0.0;
         }
       }
-      const double inv_j = 1.0 / jo;
+      const double inv_j = fast_rcp(jo);
       double qn_out[5];
 #pragma unroll
       for (int n = 0; n < 5; n++) {
