@@ -336,19 +336,27 @@ __device__ __forceinline__ void face_flux(const GasT<T>& g, const T q[6][5], T n
 // on the state itself.  The reconstructed states q_m + dq_L/R and the
 // Lax-Friedrichs flux are formed in double, so face-flux differences carry no
 // float cancellation error.
+// The Roe-average inputs of a node in the precision form: rho, u, v, w and
+// H of its double face state, rounded to T.
 template <class T>
-__device__ __forceinline__ void face_flux_mixed(const GasT<T>& gt, const GasT<double>& g,
-                                                const double q[6][5], const Side<double>& sl,
-                                                const Side<double>& sr, double nx, double ny,
-                                                double nz, double f[5]) {
+struct NodeT {
+  T r, u, v, w, H;
+};
+template <class T>
+__device__ __forceinline__ NodeT<T> node_t(const GasT<double>& g, const Side<double>& s) {
+  return {(T)s.r, (T)s.u, (T)s.v, (T)s.w, (T)side_h(g, s)};
+}
+
+// The precision-form face from the stencil's deviations d[s] = q[s] - q[2]
+// (in T), the state of node m (qm) and the Roe inputs of nodes m and m+1.
+template <class T>
+__device__ __forceinline__ void face_flux_dev(const GasT<T>& gt, const GasT<double>& g,
+                                              const T d[6][5], const double qm[5],
+                                              const NodeT<T>& nl, const NodeT<T>& nr, double nx,
+                                              double ny, double nz, double f[5]) {
   Basis<T> B;
-  roe_basis_h(gt, (T)sl.r, (T)sl.u, (T)sl.v, (T)sl.w, (T)side_h(g, sl), (T)sr.r, (T)sr.u,
-              (T)sr.v, (T)sr.w, (T)side_h(g, sr), (T)nx, (T)ny, (T)nz, B);
-  T d[6][5];
-#pragma unroll
-  for (int s = 0; s < 6; s++)
-#pragma unroll
-    for (int c = 0; c < 5; c++) d[s][c] = (T)(q[s][c] - q[2][c]);
+  roe_basis_h(gt, nl.r, nl.u, nl.v, nl.w, nl.H, nr.r, nr.u, nr.v, nr.w, nr.H, (T)nx, (T)ny,
+              (T)nz, B);
   T A[6], Bn[6];
 #pragma unroll
   for (int s = 0; s < 6; s++) {
@@ -387,14 +395,43 @@ __device__ __forceinline__ void face_flux_mixed(const GasT<T>& gt, const GasT<do
       const T m2 = B.vv * rho + B.ny * dl + B.ly * W[3] + B.my * W[4];
       const T m3 = B.ww * rho + B.nz * dl + B.lz * W[3] + B.mz * W[4];
       const T E = B.H * sm + T(0.5) * B.q2 * W[1] + B.qn * dl + B.ql * W[3] + B.qm * W[4];
-      out[side][0] = q[2][0] + (double)rho;
-      out[side][1] = q[2][1] + (double)m1;
-      out[side][2] = q[2][2] + (double)m2;
-      out[side][3] = q[2][3] + (double)m3;
-      out[side][4] = q[2][4] + (double)E;
+      out[side][0] = qm[0] + (double)rho;
+      out[side][1] = qm[1] + (double)m1;
+      out[side][2] = qm[2] + (double)m2;
+      out[side][3] = qm[3] + (double)m3;
+      out[side][4] = qm[4] + (double)E;
     }
   }
   lf_flux(g, q_to_side(g, qR), q_to_side(g, qL), nx, ny, nz, f);
+}
+
+template <class T>
+__device__ __forceinline__ void face_flux_mixed(const GasT<T>& gt, const GasT<double>& g,
+                                                const double q[6][5], const Side<double>& sl,
+                                                const Side<double>& sr, double nx, double ny,
+                                                double nz, double f[5]) {
+  T d[6][5];
+#pragma unroll
+  for (int s = 0; s < 6; s++)
+#pragma unroll
+    for (int c = 0; c < 5; c++) d[s][c] = (T)(q[s][c] - q[2][c]);
+  face_flux_dev(gt, g, d, q[2], node_t<T>(g, sl), node_t<T>(g, sr), nx, ny, nz, f);
+}
+
+// Deviations from node m of a 6-node stencil from its five consecutive
+// differences e[k] = q[k+1] - q[k] (in T): a sweep carries the differences
+// from face to face instead of forming five double differences per face.
+template <class T>
+__device__ __forceinline__ void dev_from_diffs(const T e[5][5], T d[6][5]) {
+#pragma unroll
+  for (int c = 0; c < 5; c++) {
+    d[2][c] = T(0);
+    d[1][c] = -e[1][c];
+    d[0][c] = -(e[0][c] + e[1][c]);
+    d[3][c] = e[2][c];
+    d[4][c] = e[2][c] + e[3][c];
+    d[5][c] = d[4][c] + e[4][c];
+  }
 }
 
 // face flux of the chosen arithmetic form: F = double (plain), float (precision)

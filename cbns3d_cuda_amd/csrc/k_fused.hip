@@ -173,8 +173,29 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
     t1 = fl[base + (m0 - 1) * st];
     t2 = fl[base + m0 * st];
   }
-  stage_face<F, HYB>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), t0 | t1 | t2,
+  // Precision form (float faces, not hybrid): the sweep carries the float
+  // differences e[k] = q[k+1] - q[k] of the window's nodes and the Roe
+  // inputs of nodes m (nl) and m+1 (nr), each formed once per node, instead
+  // of five double differences and two double face states per face.
+  constexpr bool DC = sizeof(F) != sizeof(double) && !HYB;
+  const GasT<F> gt{(F)c.gamma, (F)c.gam1, (F)c.gam1_inv};
+  F e[5][5];
+  NodeT<F> nl{}, nr{};
+  if constexpr (DC) {
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+#pragma unroll
+      for (int n = 0; n < 5; n++) e[k][n] = (F)(q[k + 1][n] - q[k][n]);
+    nl = node_t<F>(g, q_to_side(g, q[2]));
+    nr = node_t<F>(g, q_to_side(g, q[3]));
+    F d[6][5];
+    dev_from_diffs(e, d);
+    face_flux_dev<F>(gt, g, d, q[2], nl, nr, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz),
                      fprev);
+  } else {
+    stage_face<F, HYB>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), t0 | t1 | t2,
+                       fprev);
+  }
   for (int m = m0; m < m1; m++) {
     // shift the window by one node
 #pragma unroll
@@ -183,6 +204,16 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
       for (int n = 0; n < 5; n++) q[s][n] = q[s + 1][n];
     o = base + m * st;
     load_q(Q, S, o + 3 * st, q[5]);
+    if constexpr (DC) {
+#pragma unroll
+      for (int n = 0; n < 5; n++) {
+#pragma unroll
+        for (int k = 0; k < 4; k++) e[k][n] = e[k + 1][n];
+        e[4][n] = (F)(q[5][n] - q[4][n]);
+      }
+      nl = nr;
+      nr = node_t<F>(g, q_to_side(g, q[3]));
+    }
     const double jo = jb;  // J at node m, loaded as the previous face's right node
     // the zeta sweep's update operands are fetched before the face work so
     // their latency overlaps it; with FP64 faces and the xi-faces in this
@@ -208,10 +239,13 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
       const double* __restrict__ xm = met + XI_X * S;
       if constexpr (XSH) {
         // every lane takes part in the shuffles
+        if constexpr (!DC) {
 #pragma unroll
-        for (int s = 0; s < 6; s++)
+          for (int s = 0; s < 6; s++)
 #pragma unroll
-          for (int n = 0; n < 5; n++) qx[s][n] = s == 2 ? q[2][n] : __shfl(q[2][n], lane + s - 2);
+            for (int n = 0; n < 5; n++)
+              qx[s][n] = s == 2 ? q[2][n] : __shfl(q[2][n], lane + s - 2);
+        }
         const double jx = xm[o] * jo, jy = xm[S + o] * jo, jz = xm[2 * S + o] * jo;
         nxx = 0.5 * (jx + __shfl(jx, lane + 1));
         nxy = 0.5 * (jy + __shfl(jy, lane + 1));
@@ -236,14 +270,43 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
         nxz = 0.5 * (xm[2 * S + o] * Ja + xm[2 * S + o + 1] * Jb);
         tx = HYB && (sa.flags_x[o - 1] | sa.flags_x[o] | sa.flags_x[o + 1]);
       }
-      stage_face<F, HYB>(g, qx, nxx, nxy, nxz, tx, fxr);
+      if constexpr (XSH && DC) {
+        // the float difference to lane+1's node, the differences of lanes
+        // -2..+2 and lane+1's Roe inputs, by shuffles
+        F ex[5], ec[5][5];
+#pragma unroll
+        for (int n = 0; n < 5; n++) ex[n] = (F)(__shfl(q[2][n], lane + 1) - q[2][n]);
+#pragma unroll
+        for (int n = 0; n < 5; n++) {
+          ec[0][n] = __shfl(ex[n], lane - 2);
+          ec[1][n] = __shfl(ex[n], lane - 1);
+          ec[2][n] = ex[n];
+          ec[3][n] = __shfl(ex[n], lane + 1);
+          ec[4][n] = __shfl(ex[n], lane + 2);
+        }
+        const NodeT<F> xr{__shfl(nl.r, lane + 1), __shfl(nl.u, lane + 1), __shfl(nl.v, lane + 1),
+                          __shfl(nl.w, lane + 1), __shfl(nl.H, lane + 1)};
+        F dx[6][5];
+        dev_from_diffs(ec, dx);
+        face_flux_dev<F>(gt, g, dx, q[2], nl, xr, nxx, nxy, nxz, fxr);
+      } else {
+        stage_face<F, HYB>(g, qx, nxx, nxy, nxz, tx, fxr);
+      }
     }
     if (HYB) {
       t0 = t1;
       t1 = t2;
       t2 = fl[o + st];
     }
-    stage_face<F, HYB>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), t0 | t1 | t2, f);
+    if constexpr (DC) {
+      F d[6][5];
+      dev_from_diffs(e, d);
+      face_flux_dev<F>(gt, g, d, q[2], nl, nr, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz),
+                       f);
+    } else {
+      stage_face<F, HYB>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), t0 | t1 | t2,
+                         f);
+    }
     double dfx[5];  // xi-face difference (XI)
     if (XI) {
 #pragma unroll
