@@ -438,7 +438,7 @@ int cbns_rk3_stage(cbns_block* b, double dt, int32_t stage) {
   if ((rc = launch_fused_stage(b, dt, stage, qin, qout))) return rc;
   if (stage == 2) std::swap(b->qp, b->qspare);
   ProfScope ps(b, CBNS_K_BC);
-  return launch_bc_only(b, stage == 3 ? b->Q : b->qp);
+  return launch_bc_only(b, stage == 3 ? b->Q : b->qp, stage_prim_mask(b, stage));
 }
 
 int cbns_rk3_step(cbns_block* b, double t_end, double* t_cur, double* dt_out) {

@@ -304,7 +304,17 @@ int launch_metrics(cbns_block* b, const double* dx, const double* dy, const doub
 int launch_gradient_nodes(hipStream_t s, int IM, int JM, int KM, const double* f, double* fx,
                           double* fy, double* fz);
 int launch_bc_primitive(cbns_block* b, double* Q);
-int launch_bc_only(cbns_block* b, double* Q);
+// pmask: primitive fields whose ghost images the periodic wrap refreshes
+// (bit n: field n of rho, u, v, w, p, T; -1: all of the block's)
+int launch_bc_only(cbns_block* b, double* Q, int pmask = -1);
+// Primitives a fused stage's output needs: all of them, except inside
+// cbns_rk3_step, where the first two stages' outputs feed only the next
+// stage (the marches derive what they need from Q; the viscous flux pass
+// reads u, v, w, T) and the third stage writes all of them.
+inline int stage_prim_mask(const cbns_block* b, int stage) {
+  const int pall = b->c.viscous ? 0x3F : 0x1F;
+  return b->step_internal && stage < 3 ? (b->c.viscous ? 0x2E : 0) : pall;
+}
 int launch_primitive_all(cbns_block* b, const double* Q);
 int launch_conservative_all(cbns_block* b, double* Q);
 int launch_freestream(cbns_block* b);

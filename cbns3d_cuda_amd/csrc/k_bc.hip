@@ -200,7 +200,7 @@ __global__ void k_bc_zeta(Consts c, Box box, double* __restrict__ Q, double* __r
 // Axes are wrapped one after another over the full ghosted extent of the
 // other two, so edges and corners end up right.
 __global__ void k_wrap(Consts c, int a, double* __restrict__ Q, double* __restrict__ prim,
-                       int nprim) {
+                       int pmask) {
   const int b = (a + 1) % 3, cc = (a + 2) % 3;
   const int P[3] = {c.PX, c.PY, c.PZ};
   const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -219,7 +219,9 @@ __global__ void k_wrap(Consts c, int a, double* __restrict__ Q, double* __restri
   const int s = idx(c, ps[0], ps[1], ps[2]);
 #pragma unroll
   for (int q = 0; q < NEQ; q++) Q[(long)q * c.S + d] = Q[(long)q * c.S + s];
-  for (int q = 0; q < nprim; q++) prim[(long)q * c.S + d] = prim[(long)q * c.S + s];
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+    if (pmask >> q & 1) prim[(long)q * c.S + d] = prim[(long)q * c.S + s];
 }
 
 // All axes periodic: every ghost node copies the node at its coordinates
@@ -227,7 +229,7 @@ __global__ void k_wrap(Consts c, int a, double* __restrict__ Q, double* __restri
 // produce).  The ghost shell is enumerated slab by slab: zeta slabs (full
 // extent), then eta slabs (zeta in range), then xi slabs (eta, zeta in range).
 __global__ void k_wrap_shell(Consts c, double* __restrict__ Q, double* __restrict__ prim,
-                             int nprim) {
+                             int pmask) {
   const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned PX = c.PX, PY = c.PY, twoG = 2 * G;
   const unsigned nz = twoG * PX * PY, ny = twoG * PX * (unsigned)c.KM,
@@ -261,7 +263,9 @@ __global__ void k_wrap_shell(Consts c, double* __restrict__ Q, double* __restric
   const int d = idx(c, i, j, k), s = idx(c, si, sj, sk);
 #pragma unroll
   for (int q = 0; q < NEQ; q++) Q[(long)q * c.S + d] = Q[(long)q * c.S + s];
-  for (int q = 0; q < nprim; q++) prim[(long)q * c.S + d] = prim[(long)q * c.S + s];
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+    if (pmask >> q & 1) prim[(long)q * c.S + d] = prim[(long)q * c.S + s];
 }
 
 template <class K, class... A>
@@ -276,7 +280,7 @@ int run(cbns_block* b, const Box& box, K kern, A... args) {
 
 // Boundary rules and ghost fill for state Q whose interior primitives are
 // already in b->prim (the fused stage writes them with the new state).
-int launch_bc_only(cbns_block* b, double* Q) {
+int launch_bc_only(cbns_block* b, double* Q, int pmask) {
   const Consts& c = b->c;
   int rc;
   const int jlo = upd_lo(c, 1), jhi = upd_hi(c, 1);
@@ -297,10 +301,10 @@ int launch_bc_only(cbns_block* b, double* Q) {
     Box box{{0, 0, 0}, {c.IM, c.JM, 2}};
     if ((rc = run(b, box, k_bc_zeta, Q, b->prim))) return rc;
   }
-  const int nprim = c.viscous ? 6 : 5;
+  if (pmask < 0) pmask = c.viscous ? 0x3F : 0x1F;
   if (periodic(c, 0) && periodic(c, 1) && periodic(c, 2)) {
     const long cnt = (long)c.S - (long)c.IM * c.JM * c.KM;
-    k_wrap_shell<<<(unsigned)((cnt + 255) / 256), 256, 0, b->stream>>>(c, Q, b->prim, nprim);
+    k_wrap_shell<<<(unsigned)((cnt + 255) / 256), 256, 0, b->stream>>>(c, Q, b->prim, pmask);
     CBNS_HIP(b, hipGetLastError());
     return CBNS_OK;
   }
@@ -308,7 +312,7 @@ int launch_bc_only(cbns_block* b, double* Q) {
   for (int a = 0; a < 3; a++) {
     if (!periodic(c, a)) continue;
     const long cnt = (long)(2 * G) * P[(a + 1) % 3] * P[(a + 2) % 3];
-    k_wrap<<<(unsigned)((cnt + 255) / 256), 256, 0, b->stream>>>(c, a, Q, b->prim, nprim);
+    k_wrap<<<(unsigned)((cnt + 255) / 256), 256, 0, b->stream>>>(c, a, Q, b->prim, pmask);
     CBNS_HIP(b, hipGetLastError());
   }
   return CBNS_OK;

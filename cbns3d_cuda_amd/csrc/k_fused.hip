@@ -422,12 +422,7 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
     k_xfaces<F, HYB><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fl, fx);
     CBNS_HIP(b, hipGetLastError());
   }
-  // Primitives the stage output needs: all of them, except inside
-  // cbns_rk3_step, where the first two stages' outputs feed only the next
-  // stage (the marches derive what they need from Q; the viscous flux pass
-  // reads u, v, w, T) and the third stage writes all of them.
-  const int pall = c.viscous ? 0x3F : 0x1F;
-  const int pmask = b->step_internal && stage < 3 ? (c.viscous ? 0x2E : 0) : pall;
+  const int pmask = stage_prim_mask(b, stage);
   StageArgs sa{dt, b->dt_fold ? b->red + 2 : nullptr, stage, b->prim, b->Q, qin, qout, fx,
                b->rhs, nullptr, HYB ? fl + S : nullptr, HYB ? fl : nullptr, pmask};
   const int ni = upd_hi(c, 0) - upd_lo(c, 0) + 1;
