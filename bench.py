@@ -80,8 +80,12 @@ def kernel_work(name, p):
         return "byte", p.IM * p.JM * p.KM * DT_BYTES_PER_NODE
     if name == "indicator":  # per axis launch: 6-node stencil + 2 node states read once, 1 B out
         return "byte", sum(faces) // 3 * (5 + 8) * 8
-    # ghost shell of 4 layers: Q (5) + primitives (6) read and written
-    return "byte", ((p.IM + 8) * (p.JM + 8) * (p.KM + 8) - p.IM * p.JM * p.KM) * 11 * 16
+    # ghost shell of 4 layers: Q (5) + the primitives the stage wrote, read
+    # and written; averaged over a step (stages 1-2: u, v, w, T on viscous
+    # blocks, none on inviscid ones; stage 3: all)
+    fields = (2 * 9 + 11) / 3 if p.viscous else (2 * 5 + 10) / 3
+    shell = (p.IM + 8) * (p.JM + 8) * (p.KM + 8) - p.IM * p.JM * p.KM
+    return "byte", int(shell * fields * 16)
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_tgv256.json")
