@@ -158,6 +158,61 @@ def test_bc_reference_rules(oracle, viscous):
     assert np.array_equal(at(Q, -1, -1, 3), at(before, -1, -1, 3))
 
 
+def flat_wall_case(IM=20, JM=10, KM=7, seed=5):
+    """Cartesian mesh (eta along +y at the wall) with the reference's boundary
+    set, inviscid, random interior states."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    k, j, i = np.meshgrid(np.arange(KM, dtype=float), np.arange(JM, dtype=float),
+                          np.arange(IM, dtype=float), indexing="ij")
+    shape = i.shape
+    prim = np.stack([rng.uniform(0.6, 1.6, shape), rng.uniform(-1, 1, shape),
+                     rng.uniform(-1, 1, shape), rng.uniform(-1, 1, shape),
+                     rng.uniform(1.0, 4.0, shape)])
+    params = cases.make_params(IM, JM, KM, viscous=0, bc=(0, 0, 0), CFL=0.5, gamma=1.4,
+                               Mach=0.5, i_begin=5, i_end=IM - 4)
+    return cases.Case("flat_wall", params, i * 0.1, j * 0.08, k * 0.12, None, prim)
+
+
+def test_bc_slip_wall_ghost_energy_flat_wall(oracle):
+    """The slip wall's ghost energy (src/primitive_bc_kernel.cu:331-333) is
+    E_g = E_1 - (m_x^2 + m_y^2 - m_gx^2 + m_gy^2) / (2 rho) with the ghost
+    momentum m_g mirrored across the wall tangent t.  On a flat wall with
+    eta along +y, t = (1, 0), so m_g = (m_x, -m_y) and the expression reduces
+    to E_g = E_1 - m_y^2 / rho exactly; a sign-corrected kinetic-energy
+    balance (-m_gx^2 - m_gy^2) would leave E_g = E_1.  The mirrored state and
+    the slip node are pinned beside it."""
+    case = flat_wall_case()
+    o = oracle_block(oracle, case)
+    p = case.params
+    before = o.Q.copy()
+    o.bc_primitive()
+    Q = o.Q
+
+    def at(a, i, j, k):
+        return a[..., G + k, G + j, G + i]
+
+    checked = 0
+    for k in range(1, p.KM - 1):
+        for i in range(p.i_begin - 1, p.i_end):
+            for lj in (1, 2):
+                src = at(before, i, lj, k)
+                rho, mx, my, mz, E = src
+                ghost = at(Q, i, -lj, k)
+                assert np.array_equal(ghost[:4], np.array([rho, mx, -my, mz]))
+                assert ghost[4] == pytest.approx(E - my * my / rho, rel=1e-14, abs=1e-14)
+                assert abs(ghost[4] - E) > 1e-3 * abs(my * my / rho)
+                checked += 1
+            # the wall node takes the first interior node's state with v = 0
+            rho, mx, my, mz, E = at(before, i, 1, k)
+            u, v, w = mx / rho, my / rho, mz / rho
+            pres = 0.4 * (E - 0.5 * rho * (u * u + v * v + w * w))
+            wall = at(Q, i, 0, k)
+            assert wall[0] == rho and wall[2] == 0.0
+            assert wall[1] == pytest.approx(mx, rel=1e-14)
+            assert wall[4] == pytest.approx(pres / 0.4 + 0.5 * rho * (u * u + w * w), rel=1e-13)
+    assert checked == 2 * (p.KM - 2) * (p.i_end - p.i_begin + 1)
+
+
 # ----------------------------------------------------------- time step
 @pytest.mark.parametrize("viscous", [0, 1])
 def test_calc_dt_closed_form(oracle, viscous):

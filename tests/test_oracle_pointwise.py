@@ -191,6 +191,68 @@ def test_lf_flux_axis_aligned_is_rusanov(oracle):
         assert np.allclose(oracle.lf_flux(g, pr, pl, n), ref, rtol=1e-13, atol=1e-13)
 
 
+def _central(pl, pr, n):
+    return 0.5 * (euler_flux_n(cons(pl), n) + euler_flux_n(cons(pr), n))
+
+
+def test_lf_flux_dissipation_speed_cancels_on_a_diagonal_normal(oracle):
+    """src/lf_flux.cu:37-52 takes the dissipation speed as
+    |sum_d max_LR(|u_d| + c) n_d|, not the textbook max(|q.n| + c|n|).  With
+    |u| = |v| on each side the per-axis maxima are equal, so for n = (a, -a, 0)
+    the speed cancels to exactly 0 and the flux is the central one; the
+    textbook Rusanov speed would be positive there."""
+    g = oracle.grid_for()
+    rng = np.random.default_rng(81)
+    for _ in range(20):
+        pl, pr = rand_prim(rng), rand_prim(rng)
+        pl[2] = pl[1] * rng.choice((-1.0, 1.0))
+        pr[2] = pr[1] * rng.choice((-1.0, 1.0))
+        a = rng.uniform(0.2, 2.0)
+        n = (a, -a, 0.0)
+        f = oracle.lf_flux(g, pr, pl, n)
+        assert np.allclose(f, _central(pl, pr, n), rtol=1e-13, atol=1e-13)
+        # the textbook speed differs from zero by O(1): the check can tell them apart
+        cl, cr = np.sqrt(GAMMA * pl[4] / pl[0]), np.sqrt(GAMMA * pr[4] / pr[0])
+        nn = np.hypot(a, a)
+        rus = max(abs(pl[1] * a - pl[2] * a) + cl * nn, abs(pr[1] * a - pr[2] * a) + cr * nn)
+        jump = np.abs(cons(pr) - cons(pl)).max()
+        assert 0.5 * rus * jump > 1e-3
+
+
+def test_lf_flux_dissipation_speed_at_rest_sums_the_normal(oracle):
+    """Both states at rest with the same sound speed c: the per-axis maxima
+    are all c, so the speed is c |n_x + n_y + n_z| (the textbook speed would
+    be c |n|).  Normals with mixed signs pin the sum and its outer |.|."""
+    g = oracle.grid_for()
+    rng = np.random.default_rng(82)
+    for _ in range(20):
+        rl = rng.uniform(0.5, 2.0)
+        rr = rng.uniform(0.5, 2.0)
+        pres_l = rng.uniform(0.5, 3.0)
+        pl = np.array([rl, 0.0, 0.0, 0.0, pres_l])
+        pr = np.array([rr, 0.0, 0.0, 0.0, pres_l * rr / rl])  # same c^2 = gamma p / rho
+        c = np.sqrt(GAMMA * pres_l / rl)
+        n = rng.uniform(-1, 1, 3)
+        speed = c * abs(n.sum())
+        ref = _central(pl, pr, n) - 0.5 * speed * (cons(pr) - cons(pl))
+        assert np.allclose(oracle.lf_flux(g, pr, pl, n), ref, rtol=1e-12, atol=1e-12)
+
+
+def test_lf_flux_negative_axis_normal_dissipates(oracle):
+    """n = (-a, 0, 0): the speed is +a max(|u| + c) (the outer |.|); without it
+    the dissipation would change sign and the flux would anti-diffuse."""
+    g = oracle.grid_for()
+    rng = np.random.default_rng(83)
+    for _ in range(20):
+        pl, pr = rand_prim(rng), rand_prim(rng)
+        a = rng.uniform(0.1, 2.0)
+        n = (-a, 0.0, 0.0)
+        cl, cr = np.sqrt(GAMMA * pl[4] / pl[0]), np.sqrt(GAMMA * pr[4] / pr[0])
+        speed = a * max(abs(pl[1]) + cl, abs(pr[1]) + cr)
+        ref = _central(pl, pr, n) - 0.5 * speed * (cons(pr) - cons(pl))
+        assert np.allclose(oracle.lf_flux(g, pr, pl, n), ref, rtol=1e-13, atol=1e-13)
+
+
 # ----------------------------------------------------------- face flux
 def test_face_flux_uniform_and_linear(oracle):
     g = oracle.grid_for()
