@@ -18,6 +18,7 @@ same workload.
 import argparse
 import json
 import os
+import subprocess
 import sys
 import time
 
@@ -106,10 +107,11 @@ def roofline_of(name, ms, n, p, workload):
     avg_s = ms / n / 1e3
     if kind == "flop":
         ach = work / avg_s / 1e12
-        return {"bound": "mfma", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+        return {"bound": "valu_fp64", "achieved": ach, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": ach / PEAK_FP64_TFLOPS, "traffic": measured_traffic(name, p, workload),
                 "kernel": name, "avg_ms": ms / n,
-                "note": "FP64 VALU-bound face-flux sweep; peak = MI355X FP64 (vector = matrix)"}
+                "note": "FP64 vector-ALU-bound face-flux sweep (no matrix-core work); "
+                        "peak = MI355X FP64 vector"}
     ach = work / avg_s / 1e9
     return {"bound": "hbm", "achieved": ach, "peak": PEAK_HBM_GBS, "unit": "GB/s",
             "frac": ach / PEAK_HBM_GBS, "traffic": measured_traffic(name, p, workload),
@@ -120,8 +122,26 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """CPU model and physical core count of this host (lscpu)."""
+    info = {"model": None, "physical_cores": None, "logical_cpus": os.cpu_count()}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, check=True).stdout
+        kv = dict(line.split(":", 1) for line in out.splitlines() if ":" in line)
+        kv = {k.strip(): v.strip() for k, v in kv.items()}
+        info["model"] = kv.get("Model name")
+        info["physical_cores"] = int(kv["Core(s) per socket"]) * int(kv["Socket(s)"])
+    except (OSError, subprocess.CalledProcessError, KeyError, ValueError):
+        pass
+    return info
+
+
 def cpu_baseline(N=256, steps=3):
-    """The oracle on the host cores (bounded sample: TGV N^3, `steps` steps)."""
+    """The oracle on the host cores (bounded sample: TGV N^3, `steps` steps).
+    Its OpenMP threads are bound one per physical core (OMP_PROC_BIND=close,
+    OMP_PLACES=cores; the runtime reads them when the oracle is loaded)."""
+    os.environ.setdefault("OMP_PROC_BIND", "close")
+    os.environ.setdefault("OMP_PLACES", "cores")
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     from oracle import pyoracle as po
     from helpers import oracle_block
@@ -136,15 +156,21 @@ def cpu_baseline(N=256, steps=3):
     for _ in range(steps):
         t, _ = o.rk3_step(1e9, t)
     el = time.perf_counter() - t0
+    cpu = host_cpu()
     return {"value": N ** 3 * steps / el / 1e6, "unit": "Mcell-RK3-steps/s",
             "cores": po.lib().orc_threads(), "kind": "port",
-            "sample": "CPU oracle (C+OpenMP), viscous TGV %d^3, %d steps after 1 warm-up, %.1f s"
-                      % (N, steps, el)}
+            "cpu_model": cpu["model"], "physical_cores": cpu["physical_cores"],
+            "logical_cpus": cpu["logical_cpus"],
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")},
+            "sample": "CPU oracle (C+OpenMP, gcc -O3), viscous TGV %d^3, %d steps after 1 warm-up, "
+                      "%.1f s" % (N, steps, el)}
 
 
-def run_form(case, form, params, thresh, steps, warmup, profile=True, reuse=False):
-    """Time `steps` RK3 steps of one arithmetic form after `warmup` steps;
-    (seconds, per-kernel device times, all-finite)."""
+def run_form(case, form, params, thresh, steps, warmup, runs=1, profile_steps=0, reuse=False):
+    """`runs` timings of `steps` RK3 steps of one arithmetic form after
+    `warmup` steps, with the library's per-kernel event timing off; then
+    `profile_steps` more steps with it on for the per-kernel device times.
+    (per-run seconds, per-kernel device times, all-finite)."""
     from cbns3d_cuda_amd import cases
     from cbns3d_cuda_amd.block import BUF_Q, F_Q, FORM_HYBRID, Block
 
@@ -160,63 +186,80 @@ def run_form(case, form, params, thresh, steps, warmup, profile=True, reuse=Fals
         for _ in range(warmup):
             t, dt = b.rk3_step(1e9, t)
         b.synchronize()
-        b.kernel_times(reset=True)
-        b.set_profiling(profile)
-        b.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(steps):
-            t, dt = b.rk3_step(1e9, t)
-        b.synchronize()
-        el = time.perf_counter() - t0
-        b.set_profiling(False)
-        kt = b.kernel_times(reset=True)
+        els = []
+        for _ in range(runs):
+            b.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                t, dt = b.rk3_step(1e9, t)
+            b.synchronize()
+            els.append(time.perf_counter() - t0)
+        kt = {}
+        if profile_steps:
+            b.kernel_times(reset=True)
+            b.set_profiling(True)
+            for _ in range(profile_steps):
+                t, dt = b.rk3_step(1e9, t)
+            b.synchronize()
+            b.set_profiling(False)
+            kt = b.kernel_times(reset=True)
         finite = bool(np.isfinite(b.get_field(F_Q)).all())
     finally:
         b.free()
-    return el, kt, finite
+    return els, kt, finite
 
 
-def other_forms(case, check_case, args):
+def rate(cells, steps, els):
+    """Median over runs: (Mcell-RK3-steps/s, ms per step)."""
+    el = float(np.median(els))
+    return cells * steps / el / 1e6, 1e3 * el / steps
+
+
+def error_fields(err):
+    return {k: err[k] for k in ("L2_rho", "Linf_rho", "L2_E", "Linf_E", "troubled_fraction_mean",
+                                "diverged_step") if k in err}
+
+
+def other_forms(case, check_case, args, region=None):
     """FP32 and FP32 + learned hybrid: rates over the same timed steps, and
-    the hybrid's rho/E error against the FP64 form after `check_steps` steps
+    each form's rho/E error against the FP64 form after `check_steps` steps
     from the same initial state of `check_case`."""
     from cbns3d_cuda_amd import learned
     from cbns3d_cuda_amd.block import FORM_FP32, FORM_HYBRID
 
     p = case.params
     cells = p.IM * p.JM * p.KM
+    kms = lambda kt: {k: v[0] / max(v[1], 1) for k, v in kt.items() if v[1]}  # noqa: E731
     out = {}
-    el, kt, finite = run_form(case, FORM_FP32, None, 0.0, args.steps, args.warmup, profile=False)
-    out["fp32"] = {"value": cells * args.steps / el / 1e6, "ms_per_step": 1e3 * el / args.steps,
-                   "finite": finite}
+    els, kt, finite = run_form(case, FORM_FP32, None, 0.0, args.steps, args.warmup, args.runs,
+                               args.profile_steps)
+    v, ms = rate(cells, args.steps, els)
+    out["fp32"] = {"value": v, "ms_per_step": ms, "finite": finite, "kernel_ms": kms(kt)}
+    if args.check_steps > 0:
+        log("bench: fp32 error check, %d steps" % args.check_steps)
+        err = learned.compare(check_case, args.check_steps, FORM_FP32, log=log, region=region)
+        out["fp32"].update({"check_steps": args.check_steps, "check_case": check_case.name,
+                            "check_region": region, **error_fields(err)})
     if not os.path.exists(args.indicator):
         out["fp32_hybrid"] = {"skipped": "no indicator weights at %s" % args.indicator}
         return out
     P, thr, meta = learned.load(args.indicator)
     form = FORM_FP32 | FORM_HYBRID
-    el, kt, finite = run_form(case, form, P, thr, args.steps, args.warmup)
-    h = {"value": cells * args.steps / el / 1e6, "ms_per_step": 1e3 * el / args.steps,
-         "finite": finite, "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in kt.items() if v[1]},
-         "indicator": os.path.basename(args.indicator), "threshold": thr}
-    if args.check_steps > 0:
-        log("bench: hybrid error check, %d steps" % args.check_steps)
-        err = learned.compare(check_case, args.check_steps, form, P, thr, log=log)
-        h.update({"check_steps": args.check_steps, "check_case": check_case.name,
-                  "L2_rho": err["L2_rho"], "Linf_rho": err["Linf_rho"],
-                  "L2_E": err["L2_E"], "Linf_E": err["Linf_E"],
-                  "troubled_fraction_mean": err.get("troubled_fraction_mean"),
-                  "meets_L2_rho_1e-6": err["L2_rho"] <= 1e-6})
-    out["fp32_hybrid"] = h
-    # the same with the flags kept for floor(1/CFL) steps (options bit 2)
-    el, kt, finite = run_form(case, form, P, thr, args.steps, args.warmup, reuse=True)
-    hr = {"value": cells * args.steps / el / 1e6, "ms_per_step": 1e3 * el / args.steps,
-          "finite": finite, "indicator_every_steps": max(1, int(1.0 / case.params.CFL))}
-    if args.check_steps > 0:
-        err = learned.compare(check_case, args.check_steps, form, P, thr, log=log, reuse=True)
-        hr.update({"L2_rho": err["L2_rho"], "Linf_rho": err["Linf_rho"],
-                   "troubled_fraction_mean": err.get("troubled_fraction_mean"),
-                   "meets_L2_rho_1e-6": err["L2_rho"] <= 1e-6})
-    out["fp32_hybrid_reuse"] = hr
+    for key, reuse in (("fp32_hybrid", False), ("fp32_hybrid_reuse", True)):
+        els, kt, finite = run_form(case, form, P, thr, args.steps, args.warmup, args.runs,
+                                   args.profile_steps, reuse=reuse)
+        v, ms = rate(cells, args.steps, els)
+        h = {"value": v, "ms_per_step": ms, "finite": finite, "kernel_ms": kms(kt),
+             "indicator": os.path.basename(args.indicator), "threshold": thr}
+        if reuse:
+            h["indicator_every_steps"] = max(1, int(1.0 / case.params.CFL))
+        if args.check_steps > 0:
+            log("bench: %s error check, %d steps" % (key, args.check_steps))
+            err = learned.compare(check_case, args.check_steps, form, P, thr, log=log, reuse=reuse,
+                                  region=region)
+            h.update({"check_steps": args.check_steps, **error_fields(err),
+                      "meets_L2_rho_1e-6": err["L2_rho"] <= 1e-6})
+        out[key] = h
     return out
 
 
@@ -239,8 +282,15 @@ def main():
     ap.add_argument("--indicator", default=os.path.join(ROOT, "cbns3d_cuda_amd",
                                                         "indicator_weights.npz"))
     ap.add_argument("--check-steps", type=int, default=200,
-                    help="steps of the hybrid's error check against FP64 (0: skip)")
+                    help="steps of the FP32 and hybrid error checks against FP64 (0: skip)")
+    ap.add_argument("--runs", type=int, default=5,
+                    help="timed runs of --steps steps each; the median run is reported")
+    ap.add_argument("--profile-steps", type=int, default=0,
+                    help="steps of the per-kernel event pass after the timed runs "
+                         "(default: --steps)")
     args = ap.parse_args()
+    if args.profile_steps <= 0:
+        args.profile_steps = args.steps
     if args.gpus != 1:
         print("bench.py: runs on one GPU; the multi-GPU halo exchange is designed, "
               "not a measured path. Use --gpus 1.", file=sys.stderr)
@@ -249,6 +299,7 @@ def main():
     from cbns3d_cuda_amd import cases
 
     wl = args.workload
+    region = None
     N = args.size or {"tgv": 256, "shock_vortex": 256, "hit": 512, "slab": 512}[wl]
     if wl == "tgv":
         case = cases.taylor_green(N)
@@ -266,15 +317,19 @@ def main():
                 "(ghost layers stand in for the halo exchange)" % (N, N, N, N // 8))
     else:
         case = cases.isotropic_turbulence(N, seed=0)
-        check = cases.isotropic_turbulence(128, seed=0)
-        desc = ("decaying isotropic turbulence %d^3, Re_lambda=100, Mt=0.6, periodic; hybrid error "
-                "on a 128^3 replay" % N)
+        check = case
+        region = tuple((0, min(128, n)) for n in (N, N, N))
+        desc = ("decaying isotropic turbulence %d^3, Re_lambda=100, Mt=0.6, periodic; FP32 and "
+                "hybrid error on the 128^3 subdomain [0,128)^3 of the %d^3 run" % (N, N))
     p = case.params
     cells = p.IM * p.JM * p.KM
     log("bench: building %s" % case.name)
-    el, kt, finite = run_form(case, 0, None, 0.0, args.steps, args.warmup)
-    value = cells * args.steps / el / 1e6
-    log("bench: %.3f ms/step, %.1f Mcell-RK3-steps/s, finite=%s" % (1e3 * el / args.steps, value, finite))
+    els, kt, finite = run_form(case, 0, None, 0.0, args.steps, args.warmup, args.runs,
+                               args.profile_steps)
+    value, ms_step = rate(cells, args.steps, els)
+    log("bench: %.3f ms/step (median of %d runs: %s), %.1f Mcell-RK3-steps/s, finite=%s"
+        % (ms_step, args.runs, ", ".join("%.3f" % (1e3 * e / args.steps) for e in els), value,
+           finite))
     for k, (ms, n) in kt.items():
         if n:
             log("  %-11s %8.3f ms total  %5d launches  %.4f ms avg" % (k, ms, n, ms / n))
@@ -287,7 +342,7 @@ def main():
 
     forms = None
     if not args.no_forms:
-        forms = other_forms(case, check, args)
+        forms = other_forms(case, check, args, region)
         for k, v in forms.items():
             log("  form %s: %s" % (k, json.dumps(v)))
     del case, check
@@ -304,7 +359,9 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * el / args.steps,
+        "runs": args.runs,
+        "ms_per_step": ms_step,
+        "ms_per_step_runs": [1e3 * e / args.steps for e in els],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -316,6 +373,8 @@ def main():
         "cpu_baseline": cpu,
         "forms": forms,
         "kernel_ms": {k: v[0] / max(v[1], 1) for k, v in kt.items() if v[1]},
+        "kernel_ms_note": "HIP events on the block's stream over %d further steps after the "
+                          "timed runs (the timed runs record no events)" % args.profile_steps,
         "kernel_roofline": {k: {"frac": r["frac"], "achieved": r["achieved"], "unit": r["unit"]}
                             for k, r in per_kernel.items()},
     }

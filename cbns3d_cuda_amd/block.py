@@ -371,6 +371,19 @@ class DeviceArray:
             pass
 
 
+def calc_gradient(f):
+    """calc_gradient(info, f, fx, fy, fz) of include/calc_gradient.h:9-10: the
+    4th-order index derivatives (one-sided at the two outermost layers) of a
+    node array f of shape (KM, JM, IM), i fastest; returns (fx, fy, fz)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    KM, JM, IM = f.shape
+    out = [np.zeros_like(f) for _ in range(3)]
+    rc = lib().cbns_calc_gradient(IM, JM, KM, _ptr(f), *(_ptr(o) for o in out))
+    if rc:
+        raise CbnsError(rc, "cbns_calc_gradient failed")
+    return tuple(out)
+
+
 def gather_rows(d_src, d_idx, n, row, d_dst):
     rc = lib().cbns_gather_rows(d_src, d_idx, int(n), int(row), d_dst)
     if rc:

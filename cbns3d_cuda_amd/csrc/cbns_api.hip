@@ -1,6 +1,7 @@
 // cbns_api.hip -- extern "C" entry points of libcbns (include/cbns.h).
 // Host orchestration of the block3d_cuda launcher API (include/*.h of the
 // reference) on one HIP stream per block; status codes instead of exit().
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -402,11 +403,15 @@ int cbns_calc_numerical_flux(cbns_block* b, int32_t buf) {
 }
 
 int cbns_calc_viscous_terms(cbns_block* b) {
-  // The terms are produced together with their divergence by
-  // cbns_calc_viscous_flux_contribution (same result, one pass less).
+  // Of the reference call's outputs (gradients, mu, tau, q;
+  // src/calc_viscous_terms.cu:16-88) only mu has a field here: the
+  // gradients, stress and heat flux are formed again, with their divergence,
+  // by cbns_calc_viscous_flux_contribution and never stored.
   int rc = check_block(b);
   if (rc) return rc;
-  return ensure_unfused(b);
+  if (!b->c.viscous) return CBNS_OK;
+  if ((rc = ensure_unfused(b))) return rc;
+  return launch_sutherland(b);
 }
 
 int cbns_calc_viscous_flux_contribution(cbns_block* b) {

@@ -320,14 +320,14 @@ int launch_conservative_all(cbns_block* b, double* Q);
 int launch_freestream(cbns_block* b);
 int launch_numerical_flux(cbns_block* b, const double* Q);
 int launch_viscous_unfused(cbns_block* b);
-int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode,
-                   double* lam_part = nullptr);
+int launch_viscous(cbns_block* b, double* vf, double* out, int mode, double* lam_part = nullptr);
 int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs, double* lam_part = nullptr);
 long visc_flux_blocks(const Consts& c);
 int launch_dt_from_partials(cbns_block* b, const double* partial, int n, double cap,
                             double* out);
 int launch_update_rk3(cbns_block* b, double dt, int stage);
 int launch_calc_dt(cbns_block* b, double* dt_host);
+int launch_sutherland(cbns_block* b);
 int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout);
 bool fused_stage_uses_face_flux();
 int launch_fill(hipStream_t s, double* p, long n, double v);

@@ -57,7 +57,31 @@ __global__ void __launch_bounds__(TPB) k_dt_final(const double* __restrict__ par
   }
 }
 
+// Sutherland viscosity (unscaled) over the viscous node range [-2, N+1]^3,
+// as calc_viscous_terms leaves it (src/viscous_terms_kernel.cu:61-66)
+__global__ void __launch_bounds__(256) k_sutherland(Consts c, Box box,
+                                                    const double* __restrict__ prim,
+                                                    double* __restrict__ mu) {
+  int i, j, k;
+  if (!box_node3(box, i, j, k)) return;
+  const int o = idx(c, i, j, k);
+  const double T_l = prim[P_T * (long)c.S + o], C = c.C_T_inf;
+  mu[o] = (1.0 + C) / (T_l + C) * sqrt(T_l * T_l * T_l);
+}
+
 }  // namespace
+
+int launch_sutherland(cbns_block* b) {
+  const Consts& c = b->c;
+  Box box;
+  for (int a = 0; a < 3; a++) {
+    box.lo[a] = -2;
+    box.n[a] = axis_n(c, a) + 4;
+  }
+  k_sutherland<<<box_grid(box), box_block(), 0, b->stream>>>(c, box, b->prim, b->mu);
+  CBNS_HIP(b, hipGetLastError());
+  return CBNS_OK;
+}
 
 int launch_calc_dt(cbns_block* b, double* dt_host) {
   const Consts& c = b->c;

@@ -439,7 +439,7 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
   if (c.viscous) {
     // first stage of a folded step: the flux pass also forms dt
     double* lam = b->dt_fold && stage == 1 ? b->lam_part : nullptr;
-    int rc = sizeof(F) == sizeof(double) ? launch_viscous(b, b->vtmp, nullptr, b->rhs, 1, lam)
+    int rc = sizeof(F) == sizeof(double) ? launch_viscous(b, b->vtmp, b->rhs, 1, lam)
                                          : launch_viscous_f32(b, b->vtmp, b->rhs, lam);
     if (rc) return rc;
     if (lam) {

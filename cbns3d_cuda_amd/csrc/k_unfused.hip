@@ -95,9 +95,7 @@ int launch_numerical_flux(cbns_block* b, const double* Q) {
   return CBNS_OK;
 }
 
-int launch_viscous_unfused(cbns_block* b) {
-  return launch_viscous(b, b->vtmp, b->mu, b->dfv, 0);
-}
+int launch_viscous_unfused(cbns_block* b) { return launch_viscous(b, b->vtmp, b->dfv, 0); }
 
 int launch_update_rk3(cbns_block* b, double dt, int stage) {
   const Box ub = update_box(b->c);

@@ -5,7 +5,8 @@
 //
 //   k_visc_fluxes  per node of the viscous range [-2, N+1]^3: 4th-order
 //                  derivatives of u,v,w,T along xi,eta,zeta, Sutherland mu,
-//                  stress tau and heat flux q, contracted with J grad xi_d
+//                  stress tau and heat flux q (from the Cartesian velocity
+//                  and temperature gradients), contracted with J grad xi_d
 //                  into the 12 values Ev,Fv,Gv (4 each).  The 12 gradients,
 //                  6 stresses and 3 heat fluxes never reach HBM.
 //   k_visc_div     4th-order central divergence of Ev,Fv,Gv at the update
@@ -106,74 +107,84 @@ __device__ __forceinline__ double dwin(const double w[5]) {
   return (1.0 / 12.0) * w[0] - 2.0 / 3.0 * w[1] + (2.0 / 3.0) * w[3] - 1.0 / 12.0 * w[4];
 }
 
-// T = double (plain form) or float (precision form: the index derivatives
-// are differenced in double and rounded, stresses, heat flux and fluxes are
-// formed and stored in float).  win[f][s]: field f (u,v,w,T) at k-2+s;
-// dx[f], de[f]: d/dxi, d/deta of field f; m: the node's metrics (XI_X..JAC).
+// The node's three viscous flux vectors E[d] = J grad xi_d . [tau; u.tau - q]
+// (src/viscous_terms_kernel.cu:59-105, src/viscous_flux_kernel.cu:57-110),
+// from the Cartesian gradients
+//   g[a][b] = d u_a / d x_b = sum_d (d u_a / d xi_d) (d xi_d / d x_b)
+// of u, v, w (a = 0..2) and T (a = 3):
+//   tau = mu/Re (g + g^T - 2/3 tr(g) I),  q = -mu/(Re Pr (gamma-1) M^2) grad T.
+// du[a][d]: d u_a / d xi_d; uvw: the node's velocity; T_l: its temperature;
+// m: its metrics (XI_X..JAC).  T = double (plain form) or float (precision
+// form: the index derivatives, metrics and their products with J are rounded
+// to float, mu is formed in double).  Returns mu (unscaled).
+template <class T>
+__device__ __forceinline__ double visc_node_fluxes(const Consts& c, const double du[4][3],
+                                                   const double uvw[3], double T_l,
+                                                   const double m[10], T E[3][4]) {
+  T g[4][3];
+#pragma unroll
+  for (int a = 0; a < 4; a++) {
+    const T d0 = (T)du[a][0], d1 = (T)du[a][1], d2 = (T)du[a][2];
+#pragma unroll
+    for (int b = 0; b < 3; b++) g[a][b] = d0 * (T)m[b] + d1 * (T)m[3 + b] + d2 * (T)m[6 + b];
+  }
+  const double mu_d = (1.0 + c.C_T_inf) / (T_l + c.C_T_inf) * sqrt(T_l * T_l * T_l);
+  const T mr = (T)(mu_d * c.Re_inv);
+  const T third2 = T(2.0 / 3.0) * (g[0][0] + g[1][1] + g[2][2]);
+  T tau[3][3];
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    tau[a][a] = mr * (T(2) * g[a][a] - third2);
+#pragma unroll
+    for (int b = a + 1; b < 3; b++) tau[a][b] = tau[b][a] = mr * (g[a][b] + g[b][a]);
+  }
+  const T mq = mr * (T)c.qcoef;
+  T e[3];  // u . tau - q
+#pragma unroll
+  for (int b = 0; b < 3; b++)
+    e[b] = (T)uvw[0] * tau[0][b] + (T)uvw[1] * tau[1][b] + (T)uvw[2] * tau[2][b] - mq * g[3][b];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const T nx = (T)(m[3 * d + 0] * m[JAC]), ny = (T)(m[3 * d + 1] * m[JAC]);
+    const T nz = (T)(m[3 * d + 2] * m[JAC]);
+#pragma unroll
+    for (int a = 0; a < 3; a++) E[d][a] = nx * tau[0][a] + ny * tau[1][a] + nz * tau[2][a];
+    E[d][3] = nx * e[0] + ny * e[1] + nz * e[2];
+  }
+  return mu_d;
+}
+
+// Fluxes of node (i, j, k) stored to vf (Ev, Fv, Gv: 4 fields each).
+// win[f][s]: field f (u,v,w,T) at k-2+s; dx[f], de[f]: d/dxi, d/deta of
+// field f; m: the node's metrics.
 template <class T>
 __device__ __forceinline__ void visc_fluxes_node(const Consts& c, int i, int j, int k,
                                                  const double win[4][5],
                                                  const double dx[4], const double de[4],
                                                  const double m[10],
                                                  const double* __restrict__ prim,
-                                                 double* __restrict__ mu,
                                                  T* __restrict__ vf) {
   const long S = c.S;
   const int o = idx(c, i, j, k);
-  const double* U = prim + P_U * S + o;
-  const double* V = prim + P_V * S + o;
-  const double* W = prim + P_W * S + o;
-  const double* TT = prim + P_T * S + o;
   const int sz = c.PXY;
   // zeta: the register window, except at the one-sided outer layers of a
   // reference axis
   const bool zc = periodic(c, 2) || (k >= 0 && k <= c.KM - 1);
-  const double uz = zc ? dwin(win[0]) : dgrad(c, 2, k, U, sz);
-  const double vz = zc ? dwin(win[1]) : dgrad(c, 2, k, V, sz);
-  const double wz = zc ? dwin(win[2]) : dgrad(c, 2, k, W, sz);
-  const double tz = zc ? dwin(win[3]) : dgrad(c, 2, k, TT, sz);
-  T u_xi_l = (T)dx[0], u_eta_l = (T)de[0], u_zeta_l = (T)uz;
-  const T v_xi_l = (T)dx[1], v_eta_l = (T)de[1], v_zeta_l = (T)vz;
-  const T w_xi_l = (T)dx[2], w_eta_l = (T)de[2], w_zeta_l = (T)wz;
-  const T T_xi = (T)dx[3], T_eta = (T)de[3], T_zeta = (T)tz;
-
-  // src/viscous_terms_kernel.cu:59-105
-  const double T_l = win[3][2];
-  const double C_T_inf = c.C_T_inf;
-  const double mu_d = (1.0 + C_T_inf) / (T_l + C_T_inf) * sqrt(T_l * T_l * T_l);
-  if (mu) mu[o] = mu_d;
-  const T xi_x_l = (T)m[XI_X], xi_y_l = (T)m[XI_Y], xi_z_l = (T)m[XI_Z];
-  const T eta_x_l = (T)m[ETA_X], eta_y_l = (T)m[ETA_Y], eta_z_l = (T)m[ETA_Z];
-  const T zeta_x_l = (T)m[ZETA_X], zeta_y_l = (T)m[ZETA_Y], zeta_z_l = (T)m[ZETA_Z];
-  T mu_l = (T)(mu_d * c.Re_inv);
-  const T txx = T(2.0/3.0)*mu_l*(2*eta_x_l*u_eta_l - eta_y_l*v_eta_l - eta_z_l*w_eta_l + 2*u_xi_l*xi_x_l + 2*u_zeta_l*zeta_x_l - v_xi_l*xi_y_l - v_zeta_l*zeta_y_l - w_xi_l*xi_z_l - w_zeta_l*zeta_z_l);
-  const T txy = mu_l*(eta_x_l*v_eta_l + eta_y_l*u_eta_l + u_xi_l*xi_y_l + u_zeta_l*zeta_y_l + v_xi_l*xi_x_l + v_zeta_l*zeta_x_l);
-  const T tyy = T(2.0/3.0)*mu_l*(-eta_x_l*u_eta_l + 2*eta_y_l*v_eta_l - eta_z_l*w_eta_l - u_xi_l*xi_x_l - u_zeta_l*zeta_x_l + 2*v_xi_l*xi_y_l + 2*v_zeta_l*zeta_y_l - w_xi_l*xi_z_l - w_zeta_l*zeta_z_l);
-  const T txz = mu_l*(eta_x_l*w_eta_l + eta_z_l*u_eta_l + u_xi_l*xi_z_l + u_zeta_l*zeta_z_l + w_xi_l*xi_x_l + w_zeta_l*zeta_x_l);
-  const T tyz = mu_l*(eta_y_l*w_eta_l + eta_z_l*v_eta_l + v_xi_l*xi_z_l + v_zeta_l*zeta_z_l + w_xi_l*xi_y_l + w_zeta_l*zeta_y_l);
-  const T tzz = T(2.0/3.0)*mu_l*(-eta_x_l*u_eta_l - eta_y_l*v_eta_l + 2*eta_z_l*w_eta_l - u_xi_l*xi_x_l - u_zeta_l*zeta_x_l - v_xi_l*xi_y_l - v_zeta_l*zeta_y_l + 2*w_xi_l*xi_z_l + 2*w_zeta_l*zeta_z_l);
-  mu_l *= (T)c.qcoef;
-  const T qx = mu_l * (xi_x_l * T_xi + eta_x_l * T_eta + zeta_x_l * T_zeta);
-  const T qy = mu_l * (xi_y_l * T_xi + eta_y_l * T_eta + zeta_y_l * T_zeta);
-  const T qz = mu_l * (xi_z_l * T_xi + eta_z_l * T_eta + zeta_z_l * T_zeta);
-
-  // src/viscous_flux_kernel.cu:57-110
-  const double Jac_a = m[JAC];
-  const T uu = (T)win[0][2], vv = (T)win[1][2], ww = (T)win[2][2];
-  const T ex = uu * txx + vv * txy + ww * txz - qx;
-  const T ey = uu * txy + vv * tyy + ww * tyz - qy;
-  const T ez = uu * txz + vv * tyz + ww * tzz - qz;
+  const int pf[4] = {P_U, P_V, P_W, P_T};
+  double du[4][3];
 #pragma unroll
-  for (int d = 0; d < 3; d++) {
-    const T n_x = (T)(m[3 * d + 0] * Jac_a);
-    const T n_y = (T)(m[3 * d + 1] * Jac_a);
-    const T n_z = (T)(m[3 * d + 2] * Jac_a);
-    T* E = vf + (long)(4 * d) * S + o;
-    E[0] = n_x * txx + n_y * txy + n_z * txz;
-    E[S] = n_x * txy + n_y * tyy + n_z * tyz;
-    E[2 * S] = n_x * txz + n_y * tyz + n_z * tzz;
-    E[3 * S] = n_x * ex + n_y * ey + n_z * ez;
+  for (int f = 0; f < 4; f++) {
+    du[f][0] = dx[f];
+    du[f][1] = de[f];
+    du[f][2] = zc ? dwin(win[f]) : dgrad(c, 2, k, prim + pf[f] * S + o, sz);
   }
+  const double uvw[3] = {win[0][2], win[1][2], win[2][2]};
+  T E[3][4];
+  visc_node_fluxes<T>(c, du, uvw, win[3][2], m, E);
+#pragma unroll
+  for (int d = 0; d < 3; d++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) vf[(long)(4 * d + e) * S + o] = E[d][e];
 }
 
 // The workgroup's rows (64 x FROWS lanes, one row of j per wavefront) and
@@ -216,7 +227,6 @@ template <class T, bool LAM>
 __global__ void __launch_bounds__(64 * FROWS) k_visc_fluxes(Consts c, TileMarch tm,
                                                      const double* __restrict__ prim,
                                                      const double* __restrict__ met,
-                                                     double* __restrict__ mu,
                                                      T* __restrict__ vf,
                                                      double* __restrict__ lam_part) {
   __shared__ double plane[2][4][VPLANE_ROWS][BOX_X];
@@ -279,7 +289,7 @@ __global__ void __launch_bounds__(64 * FROWS) k_visc_fluxes(Consts c, TileMarch 
       double m[10];
 #pragma unroll
       for (int q = 0; q < 10; q++) m[q] = met[q * S + base + k * sz];
-      visc_fluxes_node<T>(c, i, j, k, win, dx, de, m, prim, mu, vf);
+      visc_fluxes_node<T>(c, i, j, k, win, dx, de, m, prim, vf);
       if (LAM && lam_line && k >= 0 && k < c.KM) {
         // u, v, w, T and the metrics from the registers, rho and p loaded
         const int o = base + k * sz;
@@ -366,22 +376,22 @@ static TileMarch visc_flux_tiles(const Consts& c) {
 
 long visc_flux_blocks(const Consts& c) { return (long)tile_march_blocks(visc_flux_tiles(c)); }
 
-// vf: 12*S scratch (of T); mu may be null.  mode 0 writes dfv (4*S) to
+// vf: 12*S scratch (of T).  mode 0 writes dfv (4*S) to
 // `out`, mode 1 subtracts the divergence from the right-hand side (5*S).
 // lam_part (visc_flux_blocks entries) or null: the flux pass also forms the
 // time step's per-workgroup spectral-radius maxima.
 template <class T>
-int viscous(cbns_block* b, T* vf, double* mu, double* out, int mode, double* lam_part) {
+int viscous(cbns_block* b, T* vf, double* out, int mode, double* lam_part) {
   const Consts& c = b->c;
   {
     ProfScope ps(b, CBNS_K_VISC_FLUX);
     const TileMarch tm = visc_flux_tiles(c);
     const dim3 grid(tile_march_blocks(tm)), block(BOX_X, FROWS);
     if (lam_part)
-      k_visc_fluxes<T, true><<<grid, block, 0, b->stream>>>(c, tm, b->prim, b->met, mu, vf,
+      k_visc_fluxes<T, true><<<grid, block, 0, b->stream>>>(c, tm, b->prim, b->met, vf,
                                                             lam_part);
     else
-      k_visc_fluxes<T, false><<<grid, block, 0, b->stream>>>(c, tm, b->prim, b->met, mu, vf,
+      k_visc_fluxes<T, false><<<grid, block, 0, b->stream>>>(c, tm, b->prim, b->met, vf,
                                                              nullptr);
     CBNS_HIP(b, hipGetLastError());
   }
@@ -397,14 +407,13 @@ int viscous(cbns_block* b, T* vf, double* mu, double* out, int mode, double* lam
   return CBNS_OK;
 }
 
-int launch_viscous(cbns_block* b, double* vf, double* mu, double* out, int mode,
-                   double* lam_part) {
-  return viscous<double>(b, vf, mu, out, mode, lam_part);
+int launch_viscous(cbns_block* b, double* vf, double* out, int mode, double* lam_part) {
+  return viscous<double>(b, vf, out, mode, lam_part);
 }
 
 // precision form: the same scratch holds 12*S floats
 int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs, double* lam_part) {
-  return viscous<float>(b, reinterpret_cast<float*>(scratch), nullptr, rhs, 1, lam_part);
+  return viscous<float>(b, reinterpret_cast<float*>(scratch), rhs, 1, lam_part);
 }
 
 }  // namespace cbns

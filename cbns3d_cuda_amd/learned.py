@@ -27,8 +27,10 @@ import time
 import numpy as np
 
 from . import cases
-from .block import (BUF_Q, F_Q, FORM_FP32, FORM_HYBRID, MLP_NPARAM, Block, DeviceArray, Mlp,
-                    gather_rows)
+from .block import (BUF_Q, F_Q, FORM_FP32, FORM_HYBRID, MLP_NPARAM, Block, CbnsError,
+                    DeviceArray, Mlp, gather_rows)
+
+ERR_NAN = 3  # CBNS_ERR_NAN
 
 ROW = 32
 
@@ -165,9 +167,13 @@ def stats_at(s, target, thr):
             "flagged_fraction": float(pred.mean()), "troubled_fraction": float(pos.mean())}
 
 
-def compare(case, steps, form, params=None, thresh=0.0, per_stage=False, log=print, reuse=False):
+def compare(case, steps, form, params=None, thresh=0.0, per_stage=False, log=print, reuse=False,
+            region=None):
     """Run `steps` RK3 steps with the plain FP64 form and with `form` from the
-    same state; L2/Linf of rho and E (updated nodes) and troubled counts."""
+    same state; L2/Linf of rho and E (updated nodes, or the node box
+    region = ((k0, k1), (j0, j1), (i0, i1)) of them) and troubled counts.  If
+    `form` produces a non-physical state (the time step is NaN) its errors
+    are inf and "diverged_step" is the step that failed."""
     out = {}
     qs = {}
     for name, f in (("fp64", 0), ("form", form)):
@@ -178,8 +184,14 @@ def compare(case, steps, form, params=None, thresh=0.0, per_stage=False, log=pri
             b.set_form(f)
             t = 0.0
             flagged = []
-            for _ in range(steps):
-                t, _ = b.rk3_step(1e9, t)
+            for n in range(steps):
+                try:
+                    t, _ = b.rk3_step(1e9, t)
+                except CbnsError as e:
+                    if name == "fp64" or e.code != ERR_NAN:
+                        raise
+                    out["diverged_step"] = n + 1
+                    break
                 if f & FORM_HYBRID:
                     flagged.append(sum(b.flag_count()))
             qs[name] = b.get_field(F_Q, 5)
@@ -193,15 +205,18 @@ def compare(case, steps, form, params=None, thresh=0.0, per_stage=False, log=pri
     p = case.params
     G = 4
     sl = [slice(None)]
-    for a, n in ((2, p.KM), (1, p.JM), (0, p.IM)):
+    for r, (a, n) in enumerate(((2, p.KM), (1, p.JM), (0, p.IM))):
         per = p.bc[a] == 1
         lo, hi = (0, n - 1) if per else (1, n - 2)
+        if region is not None:
+            lo, hi = max(lo, region[r][0]), min(hi, region[r][1] - 1)
         sl.append(slice(G + lo, G + hi + 1))
     a_, b_ = qs["form"][tuple(sl)], qs["fp64"][tuple(sl)]
     for c, nm in ((0, "rho"), (4, "E")):
         d = a_[c] - b_[c]
-        out["L2_" + nm] = float(np.sqrt(np.mean(d * d)))
-        out["Linf_" + nm] = float(np.abs(d).max())
+        diverged = "diverged_step" in out
+        out["L2_" + nm] = float("inf") if diverged else float(np.sqrt(np.mean(d * d)))
+        out["Linf_" + nm] = float("inf") if diverged else float(np.abs(d).max())
     log("compare %s form %d: %s" % (case.name, form, json.dumps(out)))
     return out
 

@@ -13,8 +13,8 @@ import numpy as np
 import pytest
 
 from cbns3d_cuda_amd import cases
-from cbns3d_cuda_amd.block import (BC_PERIODIC, BC_REF, BUF_Q, BUF_QP, F_DFV, F_FLUX, F_MET, F_PRIM,
-                                   F_Q, F_QP, Block)
+from cbns3d_cuda_amd.block import (BC_PERIODIC, BC_REF, BUF_Q, BUF_QP, F_DFV, F_FLUX, F_MET, F_MU,
+                                   F_PRIM, F_Q, F_QP, Block)
 from helpers import G, oracle_block, rel_err, update_mask
 
 pytestmark = pytest.mark.gpu
@@ -34,6 +34,11 @@ CASES = {
     "box_70x9x8_rpp_visc": lambda: cases.smooth_box(70, 9, 8, viscous=1,
                                                     bc=(BC_REF, BC_PERIODIC, BC_PERIODIC)),
     "box_7x9x6_rrr_visc": lambda: cases.smooth_box(7, 9, 6, viscous=1, bc=(BC_REF,) * 3),
+    # lines crossing the viscous sweep's tiling: 60-column tiles, 4-row
+    # tiles, 32-plane chunks
+    "box_62x9x34_visc": lambda: cases.smooth_box(62, 9, 34, viscous=1),
+    "box_63x10x35_rpr_visc": lambda: cases.smooth_box(63, 10, 35, viscous=1,
+                                                      bc=(BC_REF, BC_PERIODIC, BC_REF)),
 }
 
 
@@ -85,6 +90,21 @@ def test_calc_dt_bit_exact(pair):
     if case.params.viscous:
         o.viscous()  # fills the oracle's mu array
     assert b.calc_dt() == o.calc_dt()
+
+
+def test_viscous_terms_mu_bit_exact(pair):
+    """cbns_calc_viscous_terms leaves Sutherland's mu over the viscous node
+    range [-2, N+1]^3, as calc_viscous_terms does (src/viscous_terms_kernel.cu:61-66)."""
+    case, o, b = pair
+    if not case.params.viscous:
+        pytest.skip("inviscid block: calc_viscous_terms does nothing")
+    o.bc_primitive()
+    put_state(b, o)
+    o.viscous()
+    b.calc_viscous_terms()
+    p = case.params
+    ext = tuple(slice(G - 2, G + n + 2) for n in (p.KM, p.JM, p.IM))
+    assert np.array_equal(b.get_field(F_MU)[ext], o.mu[ext])
 
 
 def _stage_inputs(o, dt):
