@@ -23,7 +23,9 @@ BUF_Q, BUF_QP = 0, 1
 FORM_FP64, FORM_FP32, FORM_HYBRID = 0, 1, 2
 MLP_NFEAT, MLP_NPARAM = 32, 2145
 F_Q, F_QP, F_PRIM, F_MET, F_MU, F_FLUX, F_DFV = 0, 5, 10, 16, 26, 27, 42
-ERR_NAMES = {1: "ARG", 2: "HIP", 3: "NAN", 4: "MESH", 5: "STATE", 6: "NO_DEVICE"}
+ERR_NAMES = {1: "ARG", 2: "HIP", 3: "NAN", 4: "MESH", 5: "STATE", 6: "NO_DEVICE", 7: "SIZE"}
+ERR_SIZE = 7
+NOALLOC_MAX_NODES = 1 << 20
 
 
 class CbnsError(RuntimeError):
@@ -64,7 +66,7 @@ _lib = None
 # every extern "C" symbol include/cbns.h declares
 EXPORTS = [
     "cbns_create", "cbns_destroy", "cbns_last_error", "cbns_set_mesh", "cbns_get_metrics",
-    "cbns_calc_gradient", "cbns_initial_condition", "cbns_calc_conservative",
+    "cbns_calc_gradient", "cbns_calc_gradient_noalloc", "cbns_initial_condition", "cbns_calc_conservative",
     "cbns_set_conservative", "cbns_get_conservative", "cbns_calc_primitive",
     "cbns_get_primitive", "cbns_set_bc_calc_primitive", "cbns_calc_dt",
     "cbns_calc_numerical_flux", "cbns_calc_viscous_terms",
@@ -99,6 +101,7 @@ def lib():
         "cbns_set_mesh": ([B, D, D, D, D], C.c_int),
         "cbns_get_metrics": ([B] + [D] * 10, C.c_int),
         "cbns_calc_gradient": ([I, I, I, D, D, D, D], C.c_int),
+        "cbns_calc_gradient_noalloc": ([I, I, I, D, D, D, D], C.c_int),
         "cbns_initial_condition": ([B], C.c_int),
         "cbns_calc_conservative": ([B], C.c_int),
         "cbns_set_conservative": ([B, D], C.c_int),
@@ -381,6 +384,18 @@ def calc_gradient(f):
     rc = lib().cbns_calc_gradient(IM, JM, KM, _ptr(f), *(_ptr(o) for o in out))
     if rc:
         raise CbnsError(rc, "cbns_calc_gradient failed")
+    return tuple(out)
+
+
+def calc_gradient_noalloc(f):
+    """calc_gradient without device allocation: the library's fixed staging
+    array (at most NOALLOC_MAX_NODES nodes; CbnsError SIZE beyond)."""
+    f = np.ascontiguousarray(f, dtype=np.float64)
+    KM, JM, IM = f.shape
+    out = [np.zeros_like(f) for _ in range(3)]
+    rc = lib().cbns_calc_gradient_noalloc(IM, JM, KM, _ptr(f), *(_ptr(o) for o in out))
+    if rc:
+        raise CbnsError(rc, "cbns_calc_gradient_noalloc failed")
     return tuple(out)
 
 

@@ -37,7 +37,8 @@ enum {
   CBNS_ERR_NAN = 3,       /* time step is NaN (non-physical state)    */
   CBNS_ERR_MESH = 4,      /* non-positive Jacobian                    */
   CBNS_ERR_STATE = 5,     /* call order (e.g. no mesh yet)            */
-  CBNS_ERR_NO_DEVICE = 6  /* no HIP device visible                    */
+  CBNS_ERR_NO_DEVICE = 6, /* no HIP device visible                    */
+  CBNS_ERR_SIZE = 7       /* input beyond a fixed capacity            */
 };
 
 /* boundary kind per computational axis */
@@ -99,6 +100,15 @@ int cbns_get_metrics(cbns_block* blk, double* xi_x, double* xi_y, double* xi_z,
 /* calc_gradient: include/calc_gradient.h:9-10 (node arrays IM*JM*KM) */
 int cbns_calc_gradient(int32_t IM, int32_t JM, int32_t KM, const double* f,
                        double* fx, double* fy, double* fz);
+/* No counterpart in the reference: cbns_calc_gradient for callers that
+ * cannot allocate device memory (HIP graph capture, serving loops, harnesses
+ * that forbid hipMalloc).  Same arguments and results (bit for bit); the
+ * device staging is a fixed array of the library, so IM*JM*KM may not
+ * exceed CBNS_NOALLOC_MAX_NODES (else CBNS_ERR_SIZE, before any device
+ * work).  Calls that overlap on one GPU share that array and race. */
+#define CBNS_NOALLOC_MAX_NODES (1L << 20)
+int cbns_calc_gradient_noalloc(int32_t IM, int32_t JM, int32_t KM, const double* f,
+                               double* fx, double* fy, double* fz);
 
 /* initial_condition: include/initial_condition.h:9 (freestream primitives) */
 int cbns_initial_condition(cbns_block* blk);

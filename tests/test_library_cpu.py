@@ -81,3 +81,15 @@ def test_taylor_green_initial_diagnostics(oracle):
     prim = case.prim
     assert diagnostics.kinetic_energy(prim, met) == pytest.approx(0.125, rel=2e-3)
     assert diagnostics.enstrophy(prim, met) == pytest.approx(0.375, rel=5e-3)
+
+
+def test_noalloc_gradient_rejects_sizes_beyond_its_capacity():
+    """cbns_calc_gradient_noalloc refuses an input larger than its fixed
+    staging array before any device work (so this runs without a GPU)."""
+    L = block.lib()
+    n = int(round((block.NOALLOC_MAX_NODES + 1) ** (1.0 / 3.0))) + 1
+    assert n ** 3 > block.NOALLOC_MAX_NODES
+    dummy = np.zeros(1)
+    p = dummy.ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+    assert L.cbns_calc_gradient_noalloc(n, n, n, p, p, p, p) == block.ERR_SIZE
+    assert L.cbns_calc_gradient_noalloc(4, 5, 5, p, p, p, p) == 1  # CBNS_ERR_ARG
