@@ -79,7 +79,8 @@ EXPORTS = [
     "cbns_mlp_create", "cbns_mlp_destroy", "cbns_mlp_last_error", "cbns_mlp_set_params",
     "cbns_mlp_get_params", "cbns_mlp_forward", "cbns_mlp_loss_grad", "cbns_mlp_adam_step",
 ]
-KERNELS = ["xfaces", "march_eta", "march_zeta", "visc_flux", "visc_div", "bc", "dt", "indicator"]
+KERNELS = ["xfaces", "march_eta", "march_zeta", "visc_flux", "visc_div", "bc", "dt", "indicator",
+           "weno_list"]
 
 
 def lib():
@@ -304,10 +305,12 @@ class Block:
         """Features [n][32] and WENO-weight labels [n] of axis `axis`."""
         self._chk(lib().cbns_face_samples(self._h, axis, buf, d_feat, d_label))
 
-    def set_indicator(self, params, threshold=0.0, per_stage=False, count=False, reuse=False):
+    def set_indicator(self, params, threshold=0.0, per_stage=False, count=False, reuse=False,
+                      full_lists=False):
         params = np.ascontiguousarray(params, dtype=np.float32)
         assert params.size == MLP_NPARAM
-        opts = (1 if per_stage else 0) | (2 if count else 0) | (4 if reuse else 0)
+        opts = ((1 if per_stage else 0) | (2 if count else 0) | (4 if reuse else 0) |
+                (8 if full_lists else 0))
         self._chk(lib().cbns_set_indicator(self._h, _fptr(params), float(threshold), opts))
 
     def get_flags(self):

@@ -219,7 +219,8 @@ void cbns_destroy(cbns_block* b) {
                   b->red, b->lam_part};
   for (double* p : ps)
     if (p) (void)hipFree(p);
-  for (void* p : {(void*)b->flags, (void*)b->ind_P, b->ind_pack, (void*)b->ind_count})
+  for (void* p : {(void*)b->flags, (void*)b->ind_P, b->ind_pack, (void*)b->ind_count,
+                  (void*)b->tslot, (void*)b->tlist, (void*)b->tflux, (void*)b->tcount})
     if (p) (void)hipFree(p);
   if (b->h_scalar) (void)hipHostFree(b->h_scalar);
   for (auto& v : b->prof.live)
@@ -511,11 +512,37 @@ int cbns_face_samples(cbns_block* b, int32_t axis, int32_t buf, float* d_feat, f
 }
 
 int cbns_set_indicator(cbns_block* b, const float* params, float threshold, int32_t options) {
-  if (!b || !params || !std::isfinite(threshold) || options < 0 || options > 7) return CBNS_ERR_ARG;
+  if (!b || !params || !std::isfinite(threshold) || options < 0 || options > 15) return CBNS_ERR_ARG;
   CBNS_HIP(b, hipSetDevice(b->device));
   if (!b->flags) {
     CBNS_HIP(b, hipMalloc((void**)&b->flags, 3 * (size_t)b->c.S));
     CBNS_HIP(b, hipMemsetAsync(b->flags, 0, 3 * (size_t)b->c.S, b->stream));
+  }
+  // troubled-face lists: a third of each axis' faces, or all of them
+  long cap[3], total = 0;
+  for (int a = 0; a < 3; a++) {
+    const long nf = face_count(b, a);
+    cap[a] = (options & 8) ? nf : (nf + 2) / 3;
+    total += cap[a];
+  }
+  if (!b->tslot) {
+    CBNS_HIP(b, hipMalloc((void**)&b->tslot, 3 * sizeof(int) * (size_t)b->c.S));
+    CBNS_HIP(b, hipMemsetAsync(b->tslot, 0, 3 * sizeof(int) * (size_t)b->c.S, b->stream));
+    CBNS_HIP(b, hipMalloc((void**)&b->tcount, 4 * sizeof(unsigned)));
+  }
+  if (total > b->ttotal) {
+    if (b->tlist) CBNS_HIP(b, hipFree(b->tlist));
+    if (b->tflux) CBNS_HIP(b, hipFree(b->tflux));
+    b->tlist = nullptr;
+    b->tflux = nullptr;
+    CBNS_HIP(b, hipMalloc((void**)&b->tlist, sizeof(int) * (size_t)total));
+    CBNS_HIP(b, hipMalloc((void**)&b->tflux, 5 * sizeof(double) * (size_t)total));
+    b->ttotal = total;
+  }
+  // slots are positions in tflux, whose field stride stays ttotal
+  for (int a = 0; a < 3; a++) {
+    b->tcap[a] = cap[a];
+    b->toff[a] = a == 0 ? 0 : b->toff[a - 1] + cap[a - 1];
   }
   if (!b->ind_P) CBNS_HIP(b, hipMalloc((void**)&b->ind_P, 2176 * sizeof(float)));
   if (!b->ind_pack) CBNS_HIP(b, hipMalloc(&b->ind_pack, 768 * 8));

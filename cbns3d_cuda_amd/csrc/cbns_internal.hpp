@@ -149,6 +149,22 @@ __device__ inline bool box_node3(const Box& b, int& i, int& j, int& k) {
   return true;
 }
 
+// The faces a stage's flux kernels compute along axis a (face m, between
+// nodes m and m+1, stored at node m): -1..N-1 on a periodic axis, 0..N-2 on
+// a reference axis, and the update range along the other two axes; `extra`
+// widens the range along a by that many faces on either side.
+inline Box face_box(const Consts& c, int a, int extra = 0) {
+  Box box;
+  for (int d = 0; d < 3; d++) {
+    box.lo[d] = upd_lo(c, d);
+    box.n[d] = upd_hi(c, d) - upd_lo(c, d) + 1;
+  }
+  const int n = axis_n(c, a);
+  box.lo[a] = (periodic(c, a) ? -1 : 0) - extra;
+  box.n[a] = n - 1 + (periodic(c, a) ? 2 : 0) + 2 * extra;
+  return box;
+}
+
 // Tile march: 64 x 4 tiles of (i, j), each thread walking a chunk of
 // `chunk` nodes along k.  Workgroups are handed to the 8 XCDs round-robin and
 // every XCD has its own L2, so the flat block id is remapped to give each XCD
@@ -272,6 +288,19 @@ struct cbns_block {
   void* ind_pack = nullptr;      // bf16 operand tiles (3 x 256 x 8 bytes)
   unsigned* ind_count = nullptr; // troubled faces per axis [3]
   uint8_t* flags = nullptr;      // 3*S troubled-face bytes, face m at node m
+  // hybrid: the troubled faces (a flagged face or a flagged neighbour along
+  // its axis) listed per axis whenever the flags change; every stage forms
+  // their characteristic-WENO fluxes in one pass (k_weno_list) and the
+  // marches, which reconstruct linearly, take those from tflux
+  int* tslot = nullptr;          // 3*S: slot of each listed face in tflux (face m at node m)
+  int* tlist = nullptr;          // listed faces (node offsets); axis a from toff[a]
+  double* tflux = nullptr;       // 5 x ttotal: WENO flux of each listed face
+  unsigned* tcount = nullptr;    // [3] faces listed per axis (device)
+  unsigned h_tcount[3] = {0, 0, 0};
+  long tcap[3] = {0, 0, 0};      // list capacity per axis
+  long toff[3] = {0, 0, 0};      // first slot of each axis
+  long ttotal = 0;               // slots in all (tflux's field stride)
+  bool tfull = false;            // a list overflowed: the marches run WENO at every face
   std::string err;
 };
 
@@ -334,6 +363,9 @@ int launch_fill(hipStream_t s, double* p, long n, double v);
 long face_count(const cbns_block* b, int axis);
 int launch_face_samples(cbns_block* b, int axis, const double* Q, float* feat, float* label);
 int launch_indicator(cbns_block* b, const double* Q);
+// after the flags change: list the troubled faces per axis and read the
+// counts back (sets tfull when a list overflows its capacity)
+int launch_face_lists(cbns_block* b);
 int upload_indicator(cbns_block* b, const float* params);
 
 }  // namespace cbns

@@ -44,6 +44,10 @@ constexpr int CHUNK = MARCH_CHUNK;  // nodes per thread along the march axis
 #ifndef WPE_M
 #define WPE_M 2  // waves per SIMD the march kernels are compiled for
 #endif
+#ifndef WPE_H
+#define WPE_H 2  // the same for the hybrid marches (linear faces only; three
+                 // waves per SIMD measured 2-4 % slower, four spilled)
+#endif
 #ifndef FUSE_XI
 #define FUSE_XI 1  // xi-faces inside a march (no xi-face pass through HBM)
 #endif
@@ -66,17 +70,34 @@ __device__ __forceinline__ void load_q(const double* __restrict__ Q, long S, int
   for (int n = 0; n < 5; n++) q[n] = (T)Q[n * S + o];
 }
 
-// Face flux of the stage's form.  HYB: faces whose troubled flag (face m-1,
-// m or m+1 along the axis: one-cell dilation) is set run characteristic
-// WENO5, the others the linear-weight reconstruction.
+// The characteristic-WENO fluxes of a hybrid stage's troubled faces
+// (k_weno_list): face m of the axis at slot[m], field n of slot s at
+// flux[n * stride + s].
+struct TroubledFaces {
+  const int* slot;
+  const double* flux;
+  long stride;
+};
+
+// Face flux of the stage's form at face `face` (node offset).  HYB: the
+// linear-weight reconstruction, replaced by the face's characteristic-WENO
+// flux from tf where the face is troubled (its flag or a neighbour's along
+// the axis: one-cell dilation); the marches hold no WENO arithmetic, so they
+// run at the occupancy of the linear faces.  Slots read for faces outside
+// the listed range (lanes whose faces no node uses) are clamped into tflux.
 template <class F, bool HYB>
 __device__ __forceinline__ void stage_face(const GasT<double>& g, const double q[6][5],
                                            double nx, double ny, double nz, bool troubled,
-                                           double f[5]) {
-  if (!HYB || troubled) {
+                                           const TroubledFaces& tf, int face, double f[5]) {
+  if (!HYB) {
     face_flux_form<F>(g, q, nx, ny, nz, f);
-  } else {
-    face_linear<F>(g, q, nx, ny, nz, f);
+    return;
+  }
+  face_linear<F>(g, q, nx, ny, nz, f);
+  if (troubled) {
+    const long sl = min(max(tf.slot[face], 0), (int)(tf.stride - 1));
+#pragma unroll
+    for (int n = 0; n < 5; n++) f[n] = tf.flux[n * tf.stride + sl];
   }
 }
 
@@ -84,6 +105,7 @@ template <class F, bool HYB>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE_X, WPE_X))) k_xfaces(Consts c, Box box, const double* __restrict__ Q,
                                                 const double* __restrict__ met,
                                                 const uint8_t* __restrict__ flags,
+                                                TroubledFaces tf,
                                                 double* __restrict__ fx) {
   int i, j, k;
   if (!box_node3(box, i, j, k)) return;
@@ -98,7 +120,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(WPE_X,
   for (int s = 0; s < 6; s++) load_q(Q, S, a + (s - 2), q[s]);
   const GasT<double> g{c.gamma, c.gam1, c.gam1_inv};
   const bool troubled = HYB && (flags[a - 1] | flags[a] | flags[a + 1]);
-  stage_face<F, HYB>(g, q, nx, ny, nz, troubled, f);
+  stage_face<F, HYB>(g, q, nx, ny, nz, troubled, tf, a, f);
 #pragma unroll
   for (int n = 0; n < 5; n++) fx[n * S + a] = f[n];
 }
@@ -117,11 +139,15 @@ struct StageArgs {
   const uint8_t* flags; // troubled faces of the march axis (HYB)
   const uint8_t* flags_x;  // troubled xi-faces (HYB, fused xi-faces)
   int pmask;            // AX == 2: primitives to write (bit n: field n of rho, u, v, w, p, T)
+  TroubledFaces tf;     // WENO fluxes of the march axis' troubled faces (HYB)
+  TroubledFaces tf_x;   // and of the troubled xi-faces (HYB, fused xi-faces)
 };
 
 // AX = 1: march along eta, rhs = dE + dF.  AX = 2: march along zeta, finish.
 template <int AX, class F, bool HYB>
-__global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves_per_eu(WPE_M, WPE_M))) k_march(Consts c, StageArgs sa,
+__global__ void __launch_bounds__(MARCH_X * MARCH_Y)
+__attribute__((amdgpu_waves_per_eu(HYB ? WPE_H : WPE_M, HYB ? WPE_H : WPE_M)))
+k_march(Consts c, StageArgs sa,
                                                             const double* __restrict__ met) {
   constexpr bool XI = AX == xi_axis<F>() && FUSE_XI;
   // the update's time step; a NaN step (non-physical state) leaves the
@@ -194,7 +220,7 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
                      fprev);
   } else {
     stage_face<F, HYB>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), t0 | t1 | t2,
-                       fprev);
+                       sa.tf, base + (m0 - 1) * st, fprev);
   }
   // eta march: the next node's state and the next face's metrics are
   // fetched one iteration ahead (nodes up to mhi + 4 lie in the ghosted
@@ -236,7 +262,7 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
     // their latency overlaps it; with FP64 faces and the xi-faces in this
     // march they are fetched where they are used (hoisted they would push
     // the kernel past 256 VGPRs)
-    constexpr bool HOIST = AX == 2 && !(XI && sizeof(F) == sizeof(double));
+    constexpr bool HOIST = AX == 2 && !(XI && sizeof(F) == sizeof(double)) && !HYB;
     double rin[5], qnv[5];
     if (HOIST) {
 #pragma unroll
@@ -263,7 +289,16 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
       const double* __restrict__ xm = met + XI_X * S;
       if constexpr (XSH) {
         // every lane takes part in the shuffles
-        if constexpr (!DC) {
+        if constexpr (HYB) {
+          // linear xi-face from the neighbouring lanes' values, one
+          // component at a time (six values live, not the whole stencil)
+#pragma unroll
+          for (int n = 0; n < 5; n++) {
+            const double v = q[2][n];
+            linear_lr<F>(__shfl(v, lane - 2), __shfl(v, lane - 1), v, __shfl(v, lane + 1),
+                         __shfl(v, lane + 2), __shfl(v, lane + 3), qx[0][n], qx[1][n]);
+          }
+        } else if constexpr (!DC) {
 #pragma unroll
           for (int s = 0; s < 6; s++)
 #pragma unroll
@@ -313,8 +348,16 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
         F dx[6][5];
         dev_from_diffs(ec, dx);
         face_flux_dev<F>(gt, g, dx, q[2], nl, xr, nxx, nxy, nxz, fxr);
+      } else if constexpr (XSH && HYB) {
+        // qx[0], qx[1]: the reconstructed left and right states
+        lf_flux(g, q_to_side(g, qx[1]), q_to_side(g, qx[0]), nxx, nxy, nxz, fxr);
+        if (tx) {
+          const long sl = min(max(sa.tf_x.slot[o], 0), (int)(sa.tf_x.stride - 1));
+#pragma unroll
+          for (int n = 0; n < 5; n++) fxr[n] = sa.tf_x.flux[n * sa.tf_x.stride + sl];
+        }
       } else {
-        stage_face<F, HYB>(g, qx, nxx, nxy, nxz, tx, fxr);
+        stage_face<F, HYB>(g, qx, nxx, nxy, nxz, tx, sa.tf_x, o, fxr);
       }
     }
     if (HYB) {
@@ -329,7 +372,7 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
                        f);
     } else {
       stage_face<F, HYB>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), t0 | t1 | t2,
-                         f);
+                         sa.tf, o, f);
     }
     double dfx[5];  // xi-face difference (XI)
     if (XI) {
@@ -390,7 +433,113 @@ __global__ void __launch_bounds__(MARCH_X * MARCH_Y) __attribute__((amdgpu_waves
   }
 }
 
+// The troubled faces of axis AX in the flux kernels' face range fb: face m
+// is troubled when its flag or a neighbour's along AX is set (the same
+// one-cell dilation the marches apply).  Each is appended to list (node
+// offset) and its slot (slot_base + position) recorded in slot[face]; a
+// wavefront reserves its positions with one atomic.  Positions past cap are
+// counted but not written (the host then runs full WENO).
+template <int AX>
+__global__ void __launch_bounds__(256) k_face_list(Consts c, Box fb,
+                                                   const uint8_t* __restrict__ fl,
+                                                   int* __restrict__ slot, int* __restrict__ list,
+                                                   unsigned* __restrict__ count, unsigned cap,
+                                                   int slot_base) {
+  int i = 0, j = 0, k = 0;
+  const bool valid = box_node3(fb, i, j, k);
+  int a = 0;
+  bool tr = false;
+  if (valid) {
+    a = idx(c, i, j, k);
+    const int st = stride(c, AX);
+    tr = (fl[a - st] | fl[a] | fl[a + st]) != 0;
+  }
+  const unsigned long long bal = __ballot(tr);
+  if (bal == 0ull) return;  // the whole wavefront
+  const int lane = threadIdx.x;  // one wavefront per row of the 64 x 4 block
+  const int first = __ffsll((long long)bal) - 1;
+  unsigned pos0 = 0;
+  if (lane == first) pos0 = atomicAdd(count, (unsigned)__popcll(bal));
+  pos0 = __shfl(pos0, first);
+  const unsigned pos = pos0 + (unsigned)__popcll(bal & ((1ull << lane) - 1ull));
+  if (tr && pos < cap) {
+    list[pos] = a;
+    slot[a] = slot_base + (int)pos;
+  }
+}
+
+// Characteristic-WENO flux (the stage's arithmetic form) of each listed face
+// of axis AX, from the stage input Q, into its slot.
+template <int AX, class F>
+__global__ void __launch_bounds__(256) k_weno_list(Consts c, const int* __restrict__ list,
+                                                   unsigned n, const double* __restrict__ Q,
+                                                   const double* __restrict__ met,
+                                                   double* __restrict__ flux, long fstride,
+                                                   int slot_base) {
+  const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const long S = c.S;
+  const int st = stride(c, AX);
+  const int a = list[t], bn = a + st;
+  double q[6][5], f[5];
+#pragma unroll
+  for (int s = 0; s < 6; s++) load_q(Q, S, a + (s - 2) * st, q[s]);
+  const double Ja = met[JAC * S + a], Jb = met[JAC * S + bn];
+  const double* nX = met + (3 * AX + 0) * S;
+  const double* nY = met + (3 * AX + 1) * S;
+  const double* nZ = met + (3 * AX + 2) * S;
+  const double ax = nX[a] * Ja, ay = nY[a] * Ja, az = nZ[a] * Ja;
+  const double bx = nX[bn] * Jb, by = nY[bn] * Jb, bz = nZ[bn] * Jb;
+  const GasT<double> g{c.gamma, c.gam1, c.gam1_inv};
+  face_flux_form<F>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), f);
+#pragma unroll
+  for (int m = 0; m < 5; m++) flux[m * fstride + slot_base + t] = f[m];
+}
+
+template <int AX>
+void launch_face_list_axis(cbns_block* b, const Box& fb) {
+  k_face_list<AX><<<box_grid(fb), box_block(), 0, b->stream>>>(
+      b->c, fb, b->flags + (long)AX * b->c.S, b->tslot + (long)AX * b->c.S, b->tlist + b->toff[AX],
+      b->tcount + AX, (unsigned)b->tcap[AX], (int)b->toff[AX]);
+}
+
+template <class F>
+int launch_weno_lists(cbns_block* b, const double* qin) {
+  ProfScope ps(b, CBNS_K_WENO_LIST);
+  for (int a = 0; a < 3; a++) {
+    const unsigned n = b->h_tcount[a];
+    if (n == 0) continue;
+    const unsigned nb = (n + 255) / 256;
+    const int sb = (int)b->toff[a];
+    const int* lst = b->tlist + b->toff[a];
+    if (a == 0) k_weno_list<0, F><<<nb, 256, 0, b->stream>>>(b->c, lst, n, qin, b->met, b->tflux, b->ttotal, sb);
+    else if (a == 1) k_weno_list<1, F><<<nb, 256, 0, b->stream>>>(b->c, lst, n, qin, b->met, b->tflux, b->ttotal, sb);
+    else k_weno_list<2, F><<<nb, 256, 0, b->stream>>>(b->c, lst, n, qin, b->met, b->tflux, b->ttotal, sb);
+    CBNS_HIP(b, hipGetLastError());
+  }
+  return CBNS_OK;
+}
+
 }  // namespace
+
+int launch_face_lists(cbns_block* b) {
+  CBNS_HIP(b, hipMemsetAsync(b->tcount, 0, 3 * sizeof(unsigned), b->stream));
+  for (int a = 0; a < 3; a++) {
+    const Box fb = face_box(b->c, a, 0);
+    if (fb.count() == 0) continue;
+    if (a == 0) launch_face_list_axis<0>(b, fb);
+    else if (a == 1) launch_face_list_axis<1>(b, fb);
+    else launch_face_list_axis<2>(b, fb);
+    CBNS_HIP(b, hipGetLastError());
+  }
+  CBNS_HIP(b, hipMemcpyAsync(b->h_tcount, b->tcount, 3 * sizeof(unsigned), hipMemcpyDeviceToHost,
+                             b->stream));
+  CBNS_HIP(b, hipStreamSynchronize(b->stream));
+  b->tfull = false;
+  for (int a = 0; a < 3; a++)
+    if ((long)b->h_tcount[a] > b->tcap[a]) b->tfull = true;
+  return CBNS_OK;
+}
 
 template <class F, bool HYB>
 int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout) {
@@ -403,10 +552,15 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
                      (stage == 1 && b->ind_age >= b->ind_every);
     if (due) {
       int rc = launch_indicator(b, qin);
+      if (!rc) rc = launch_face_lists(b);
       if (rc) return rc;
       b->ind_age = 0;
     }
     if (stage == 1) b->ind_age++;
+    // more troubled faces than the lists hold: WENO at every face
+    if (b->tfull) return fused_stage<F, false>(b, dt, stage, qin, qout);
+    int rc = launch_weno_lists<F>(b, qin);
+    if (rc) return rc;
   }
   const long S = c.S;
   const uint8_t* fl = HYB ? b->flags : nullptr;
@@ -419,12 +573,15 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
     box.lo[0] = periodic(c, 0) ? -1 : 0;
     box.n[0] = c.IM - 1 + (periodic(c, 0) ? 2 : 0);  // faces -1..N-1 or 0..N-2
     ProfScope ps(b, CBNS_K_XFACES);
-    k_xfaces<F, HYB><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fl, fx);
+    const TroubledFaces tx{b->tslot, b->tflux, b->ttotal};
+    k_xfaces<F, HYB><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fl, tx, fx);
     CBNS_HIP(b, hipGetLastError());
   }
   const int pmask = stage_prim_mask(b, stage);
   StageArgs sa{dt, b->dt_fold ? b->red + 2 : nullptr, stage, b->prim, b->Q, qin, qout, fx,
-               b->rhs, nullptr, HYB ? fl + S : nullptr, HYB ? fl : nullptr, pmask};
+               b->rhs, nullptr, HYB ? fl + S : nullptr, HYB ? fl : nullptr, pmask,
+               TroubledFaces{HYB ? b->tslot + S : nullptr, b->tflux, b->ttotal},
+               TroubledFaces{HYB ? b->tslot : nullptr, b->tflux, b->ttotal}};
   const int ni = upd_hi(c, 0) - upd_lo(c, 0) + 1;
   {
     const int nt = upd_hi(c, 2) - upd_lo(c, 2) + 1;
@@ -455,7 +612,10 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
     dim3 grid((ni + per_wave - 1) / per_wave, (nt + MARCH_Y - 1) / MARCH_Y,
               (nm + CHUNK - 1) / CHUNK);
     ProfScope ps(b, CBNS_K_MARCH_ZETA);
-    if (HYB) sa.flags = fl + 2 * S;
+    if (HYB) {
+      sa.flags = fl + 2 * S;
+      sa.tf.slot = b->tslot + 2 * S;
+    }
     k_march<2, F, HYB><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
     CBNS_HIP(b, hipGetLastError());
   }

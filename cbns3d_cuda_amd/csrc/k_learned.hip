@@ -355,18 +355,6 @@ __global__ void k_zero_counts(unsigned* cnt) {
 // ---- block side -----------------------------------------------------------
 
 // faces of axis `a` the flux kernels compute: -1..N-1 (periodic) or 0..N-2
-static Box face_box(const Consts& c, int a, int extra) {
-  Box box;
-  for (int d = 0; d < 3; d++) {
-    box.lo[d] = upd_lo(c, d);
-    box.n[d] = upd_hi(c, d) - upd_lo(c, d) + 1;
-  }
-  const int n = axis_n(c, a);
-  box.lo[a] = (periodic(c, a) ? -1 : 0) - extra;
-  box.n[a] = n - 1 + (periodic(c, a) ? 2 : 0) + 2 * extra;
-  return box;
-}
-
 long face_count(const cbns_block* b, int axis) { return face_box(b->c, axis, 0).count(); }
 
 int launch_face_samples(cbns_block* b, int axis, const double* Q, float* feat, float* label) {

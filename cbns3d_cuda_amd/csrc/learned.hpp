@@ -143,25 +143,29 @@ __device__ __forceinline__ double face_label(const GasT<double>& g, const double
 // in double (as face_flux_mixed).
 //   left  = (2 q0 - 13 q1 + 47 q2 + 27 q3 - 3 q4) / 60
 //   right = (2 q5 - 13 q4 + 47 q3 + 27 q2 - 3 q1) / 60
+// one component: the left and right states of the face from its six
+// stencil values v0..v5
+template <class F>
+__device__ __forceinline__ void linear_lr(double v0, double v1, double v2, double v3, double v4,
+                                          double v5, double& L, double& R) {
+  if constexpr (sizeof(F) == sizeof(double)) {
+    L = (2.0 * v0 - 13.0 * v1 + 47.0 * v2 + 27.0 * v3 - 3.0 * v4) * (1.0 / 60.0);
+    R = (2.0 * v5 - 13.0 * v4 + 47.0 * v3 + 27.0 * v2 - 3.0 * v1) * (1.0 / 60.0);
+  } else {
+    const F d0 = (F)(v0 - v2), d1 = (F)(v1 - v2), d3 = (F)(v3 - v2), d4 = (F)(v4 - v2);
+    const F d5 = (F)(v5 - v2);
+    L = v2 + (double)((F(2) * d0 - F(13) * d1 + F(27) * d3 - F(3) * d4) * F(1.0 / 60.0));
+    R = v2 + (double)((F(2) * d5 - F(13) * d4 + F(47) * d3 - F(3) * d1) * F(1.0 / 60.0));
+  }
+}
+
 template <class F>
 __device__ __forceinline__ void face_linear(const GasT<double>& g, const double q[6][5], double nx,
                                             double ny, double nz, double f[5]) {
   double qL[5], qR[5];
 #pragma unroll
-  for (int c = 0; c < 5; c++) {
-    if constexpr (sizeof(F) == sizeof(double)) {
-      qL[c] = (2.0 * q[0][c] - 13.0 * q[1][c] + 47.0 * q[2][c] + 27.0 * q[3][c] - 3.0 * q[4][c]) *
-              (1.0 / 60.0);
-      qR[c] = (2.0 * q[5][c] - 13.0 * q[4][c] + 47.0 * q[3][c] + 27.0 * q[2][c] - 3.0 * q[1][c]) *
-              (1.0 / 60.0);
-    } else {
-      const F d0 = (F)(q[0][c] - q[2][c]), d1 = (F)(q[1][c] - q[2][c]);
-      const F d3 = (F)(q[3][c] - q[2][c]), d4 = (F)(q[4][c] - q[2][c]);
-      const F d5 = (F)(q[5][c] - q[2][c]);
-      qL[c] = q[2][c] + (double)((F(2) * d0 - F(13) * d1 + F(27) * d3 - F(3) * d4) * F(1.0 / 60.0));
-      qR[c] = q[2][c] + (double)((F(2) * d5 - F(13) * d4 + F(47) * d3 - F(3) * d1) * F(1.0 / 60.0));
-    }
-  }
+  for (int c = 0; c < 5; c++)
+    linear_lr<F>(q[0][c], q[1][c], q[2][c], q[3][c], q[4][c], q[5][c], qL[c], qR[c]);
   lf_flux(g, q_to_side(g, qR), q_to_side(g, qL), nx, ny, nz, f);
 }
 

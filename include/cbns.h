@@ -170,7 +170,8 @@ enum {
   CBNS_K_BC = 5,         /* boundary / ghost fill                   */
   CBNS_K_DT = 6,         /* CFL time step reduction                 */
   CBNS_K_INDICATOR = 7,  /* learned troubled-face flags (hybrid)    */
-  CBNS_K_COUNT = 8
+  CBNS_K_WENO_LIST = 8,  /* WENO fluxes of the troubled faces (hybrid) */
+  CBNS_K_COUNT = 9
 };
 int cbns_set_profiling(cbns_block* blk, int32_t on);
 int cbns_kernel_time(cbns_block* blk, int32_t kernel, int32_t reset, double* ms, int64_t* launches);
@@ -202,7 +203,11 @@ int cbns_face_samples(cbns_block* blk, int32_t axis, int32_t buf, float* d_feat,
  * (default: once per step, on the stage-1 input); bit 1: count troubled faces
  * (cbns_flag_count); bit 2: keep the flags for floor(1/CFL) steps (a wave
  * moves at most CFL cells per step along each axis, so within that many steps
- * it stays inside the one-cell dilation). */
+ * it stays inside the one-cell dilation).  The troubled faces of each axis
+ * are listed when the flags change and their WENO fluxes formed in one pass
+ * per stage; lists hold a third of an axis' faces, and a step whose
+ * troubled faces exceed that runs WENO at every face (the plain form's
+ * marches) -- bit 3: lists that hold every face (never that fallback). */
 int cbns_set_indicator(cbns_block* blk, const float* params, float threshold, int32_t options);
 /* troubled faces (before dilation) per axis found by the last indicator pass
  * among the faces the flux kernels compute (cbns_face_count); needs options

@@ -8,8 +8,10 @@ orc_rk3_stage_hybrid):
   training samples     features (float) and labels (FP64 weights, stored float)
                        of every face vs the oracle's, per axis
   hybrid stage         with the GPU's own troubled-face bytes, the FP64 hybrid
-                       stage vs the oracle's hybrid stage (1e-12 relative, the
-                       plain-form bar); every face troubled -> the plain stage;
+                       stage (linear marches + listed WENO faces) vs the
+                       oracle's hybrid stage (1e-12 relative, the plain-form
+                       bar); every face troubled -> the plain stage (through
+                       the lists, and through the full-WENO fallback);
                        no face troubled -> the oracle's all-linear stage; the
                        FP32 hybrid within the FP32 form's 1e-7
   flags                the GPU's flags are logit > threshold of the oracle's
@@ -114,7 +116,7 @@ def test_face_samples_match_oracle(oracle, builder):
         b.free()
 
 
-def _stage_pair(oracle, case, params, thresh, form):
+def _stage_pair(oracle, case, params, thresh, form, full_lists=True):
     o = oracle_block(oracle, case)
     o.bc_primitive()
     if case.params.viscous:
@@ -122,7 +124,7 @@ def _stage_pair(oracle, case, params, thresh, form):
     dt = o.calc_dt()
     b = Block(case.params)
     b.calc_metrics(case.x, case.y, case.z, case.period)
-    b.set_indicator(params, thresh, count=True)
+    b.set_indicator(params, thresh, count=True, full_lists=full_lists)
     b.set_form(form)
     b.set_field(F_Q, o.Q)
     b.set_field(F_QP, o.Qp)
@@ -183,13 +185,17 @@ def face_values(b, arr, axis):
     return arr[lo[2] + G:hi[2] + G + 1, lo[1] + G:hi[1] + G + 1, lo[0] + G:hi[0] + G + 1].ravel()
 
 
-@pytest.mark.parametrize("bias,expect", [(100.0, "weno"), (-100.0, "linear")])
-def test_hybrid_extremes(oracle, bias, expect):
+@pytest.mark.parametrize("bias,expect,full_lists", [(100.0, "weno", True), (100.0, "weno", False),
+                                                    (-100.0, "linear", True)])
+def test_hybrid_extremes(oracle, bias, expect, full_lists):
+    """Every face troubled: the listed WENO faces (lists that hold every face)
+    or the fallback to WENO at every face (lists of a third of the faces)
+    give the plain stage; no face troubled: the all-linear stage."""
     case = cases.isentropic_vortex(16)
-    print("case", case.name, "bias", bias, flush=True)
+    print("case", case.name, "bias", bias, "full lists", full_lists, flush=True)
     P = np.zeros(MLP_NPARAM, dtype=np.float32)
     P[2144] = bias
-    o, b, dt = _stage_pair(oracle, case, P, 0.0, FORM_HYBRID)
+    o, b, dt = _stage_pair(oracle, case, P, 0.0, FORM_HYBRID, full_lists)
     try:
         flags = b.get_flags()
         if expect == "weno":
@@ -233,7 +239,7 @@ def test_flag_reuse_over_steps(oracle):
     b = Block(case.params)
     try:
         b.calc_metrics(case.x, case.y, case.z, case.period)
-        b.set_indicator(P, 0.0, count=True, reuse=True)
+        b.set_indicator(P, 0.0, count=True, reuse=True, full_lists=True)
         b.set_form(FORM_HYBRID)
         b.set_profiling(True)
         b.set_field(F_Q, o.Q)

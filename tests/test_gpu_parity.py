@@ -39,6 +39,8 @@ CASES = {
     "box_62x9x34_visc": lambda: cases.smooth_box(62, 9, 34, viscous=1),
     "box_63x10x35_rpr_visc": lambda: cases.smooth_box(63, 10, 35, viscous=1,
                                                       bc=(BC_REF, BC_PERIODIC, BC_REF)),
+    # centred divergence on node pairs: lines of 66 nodes (one paired tile)
+    "box_66x9x8_centred_visc": lambda: cases.smooth_box(66, 9, 8, viscous=1, visc_centered=1),
 }
 
 
