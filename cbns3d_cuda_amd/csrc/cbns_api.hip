@@ -22,7 +22,7 @@ int hip_fail(cbns_block* b, hipError_t e, const char* what) {
   return fail(b, CBNS_ERR_HIP, m);
 }
 
-void prof_begin(cbns_block* b, int id, std::pair<hipEvent_t, hipEvent_t>& ev) {
+void prof_begin(cbns_block* b, int id, std::pair<hipEvent_t, hipEvent_t>& ev, hipStream_t s) {
   if (!b->prof.on) return;
   if (b->prof.pool.empty()) {
     hipEvent_t a = nullptr, c = nullptr;
@@ -31,14 +31,19 @@ void prof_begin(cbns_block* b, int id, std::pair<hipEvent_t, hipEvent_t>& ev) {
   }
   ev = b->prof.pool.back();
   b->prof.pool.pop_back();
-  (void)hipEventRecord(ev.first, b->stream);
+  (void)hipEventRecord(ev.first, s);
   (void)id;
 }
 
-void prof_end(cbns_block* b, int id, std::pair<hipEvent_t, hipEvent_t>& ev) {
+void prof_end(cbns_block* b, int id, std::pair<hipEvent_t, hipEvent_t>& ev, hipStream_t s) {
   if (!b->prof.on || !ev.first) return;
-  (void)hipEventRecord(ev.second, b->stream);
+  (void)hipEventRecord(ev.second, s);
   b->prof.live[id].push_back(ev);
+}
+
+ProfScope::ProfScope(cbns_block* b_, int id_, hipStream_t s_)
+    : b(b_), id(id_), s(s_ ? s_ : b_->stream) {
+  prof_begin(b, id, ev, s);
 }
 
 namespace {
@@ -229,6 +234,10 @@ void cbns_destroy(cbns_block* b) {
     (void)hipEventDestroy(ev.first);
     (void)hipEventDestroy(ev.second);
   }
+  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
+  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
+  if (b->stream2) (void)hipStreamDestroy(b->stream2);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -517,6 +526,11 @@ int cbns_set_indicator(cbns_block* b, const float* params, float threshold, int3
   if (!b->flags) {
     CBNS_HIP(b, hipMalloc((void**)&b->flags, 3 * (size_t)b->c.S));
     CBNS_HIP(b, hipMemsetAsync(b->flags, 0, 3 * (size_t)b->c.S, b->stream));
+  }
+  if (!b->stream2) {
+    CBNS_HIP(b, hipStreamCreate(&b->stream2));
+    CBNS_HIP(b, hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
+    CBNS_HIP(b, hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
   }
   // troubled-face lists: a third of each axis' faces, or all of them
   long cap[3], total = 0;

@@ -253,6 +253,9 @@ struct cbns_block {
   cbns::Consts c{};
   int device = 0;
   hipStream_t stream = nullptr;
+  // second stream (hybrid form): the indicator beside the viscous flux pass
+  hipStream_t stream2 = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool have_mesh = false;
   int form = CBNS_FORM_FP64;
   // device arrays
@@ -317,14 +320,15 @@ int hip_fail(cbns_block* b, hipError_t e, const char* what);
   } while (0)
 
 // event bracket around one kernel launch (no-op unless profiling)
-void prof_begin(cbns_block* b, int id, std::pair<hipEvent_t, hipEvent_t>& ev);
-void prof_end(cbns_block* b, int id, std::pair<hipEvent_t, hipEvent_t>& ev);
+void prof_begin(cbns_block* b, int id, std::pair<hipEvent_t, hipEvent_t>& ev, hipStream_t s);
+void prof_end(cbns_block* b, int id, std::pair<hipEvent_t, hipEvent_t>& ev, hipStream_t s);
 struct ProfScope {
   cbns_block* b;
   int id;
+  hipStream_t s;
   std::pair<hipEvent_t, hipEvent_t> ev{nullptr, nullptr};
-  ProfScope(cbns_block* b_, int id_) : b(b_), id(id_) { prof_begin(b, id, ev); }
-  ~ProfScope() { prof_end(b, id, ev); }
+  ProfScope(cbns_block* b_, int id_, hipStream_t s_ = nullptr);
+  ~ProfScope() { prof_end(b, id, ev, s); }
 };
 
 // kernel launchers (one translation unit each)
@@ -349,8 +353,11 @@ int launch_conservative_all(cbns_block* b, double* Q);
 int launch_freestream(cbns_block* b);
 int launch_numerical_flux(cbns_block* b, const double* Q);
 int launch_viscous_unfused(cbns_block* b);
-int launch_viscous(cbns_block* b, double* vf, double* out, int mode, double* lam_part = nullptr);
-int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs, double* lam_part = nullptr);
+// parts: 1 the flux pass, 2 the divergence pass, 3 both
+int launch_viscous(cbns_block* b, double* vf, double* out, int mode, double* lam_part = nullptr,
+                   int parts = 3);
+int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs, double* lam_part = nullptr,
+                       int parts = 3);
 long visc_flux_blocks(const Consts& c);
 int launch_dt_from_partials(cbns_block* b, const double* partial, int n, double cap,
                             double* out);
@@ -362,7 +369,7 @@ bool fused_stage_uses_face_flux();
 int launch_fill(hipStream_t s, double* p, long n, double v);
 long face_count(const cbns_block* b, int axis);
 int launch_face_samples(cbns_block* b, int axis, const double* Q, float* feat, float* label);
-int launch_indicator(cbns_block* b, const double* Q);
+int launch_indicator(cbns_block* b, const double* Q, hipStream_t s);
 // after the flags change: list the troubled faces per axis and read the
 // counts back (sets tfull when a list overflows its capacity)
 int launch_face_lists(cbns_block* b);

@@ -542,28 +542,83 @@ int launch_face_lists(cbns_block* b) {
 }
 
 template <class F, bool HYB>
-int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout) {
+int launch_marches_eta(cbns_block* b, StageArgs& sa) {
+  const Consts& c = b->c;
+  const int ni = upd_hi(c, 0) - upd_lo(c, 0) + 1;
+  const int nt = upd_hi(c, 2) - upd_lo(c, 2) + 1;
+  const int nm = upd_hi(c, 1) - upd_lo(c, 1) + 1;
+  const int per_wave = march_width<1, F>();
+  dim3 grid((ni + per_wave - 1) / per_wave, (nt + MARCH_Y - 1) / MARCH_Y, (nm + CHUNK - 1) / CHUNK);
+  ProfScope ps(b, CBNS_K_MARCH_ETA);
+  k_march<1, F, HYB><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
+  CBNS_HIP(b, hipGetLastError());
+  return CBNS_OK;
+}
+
+template <class F, bool HYB>
+int launch_marches_zeta(cbns_block* b, StageArgs& sa) {
+  const Consts& c = b->c;
+  const int ni = upd_hi(c, 0) - upd_lo(c, 0) + 1;
+  const int nt = upd_hi(c, 1) - upd_lo(c, 1) + 1;
+  const int nm = upd_hi(c, 2) - upd_lo(c, 2) + 1;
+  const int per_wave = march_width<2, F>();
+  dim3 grid((ni + per_wave - 1) / per_wave, (nt + MARCH_Y - 1) / MARCH_Y, (nm + CHUNK - 1) / CHUNK);
+  ProfScope ps(b, CBNS_K_MARCH_ZETA);
+  k_march<2, F, HYB><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
+  CBNS_HIP(b, hipGetLastError());
+  return CBNS_OK;
+}
+
+template <class F>
+int viscous_part(cbns_block* b, double* lam, int parts) {
+  return sizeof(F) == sizeof(double) ? launch_viscous(b, b->vtmp, b->rhs, 1, lam, parts)
+                                     : launch_viscous_f32(b, b->vtmp, b->rhs, lam, parts);
+}
+
+// One fused stage.  Order: [indicator (hybrid, when due)] + viscous flux
+// pass, eta march, viscous divergence, zeta march.  The flux pass reads only
+// the stage input's primitives, so it runs before the eta march; on a viscous
+// hybrid stage whose indicator is due, the indicator (bound by its
+// arithmetic) runs on the block's second stream beside the flux pass (bound
+// by memory), and the face lists wait for both.
+template <class F>
+int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout,
+                bool hyb_form) {
   const Consts& c = b->c;
   double* fx = b->flux;  // first 5 face arrays
-  if (HYB) {
+  int rc = CBNS_OK;
+  bool side = false;
+  if (hyb_form) {
     // once per step on the stage-1 input (or every ind_every steps, or at
     // every stage)
     const bool due = !b->flags_valid || b->ind_per_stage ||
                      (stage == 1 && b->ind_age >= b->ind_every);
     if (due) {
-      int rc = launch_indicator(b, qin);
-      if (!rc) rc = launch_face_lists(b);
-      if (rc) return rc;
+      if (c.viscous && b->stream2) {
+        CBNS_HIP(b, hipEventRecord(b->ev_fork, b->stream));
+        CBNS_HIP(b, hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
+        if ((rc = launch_indicator(b, qin, b->stream2))) return rc;
+        CBNS_HIP(b, hipEventRecord(b->ev_join, b->stream2));
+        side = true;
+      } else if ((rc = launch_indicator(b, qin, b->stream)) || (rc = launch_face_lists(b))) {
+        return rc;
+      }
       b->ind_age = 0;
     }
     if (stage == 1) b->ind_age++;
-    // more troubled faces than the lists hold: WENO at every face
-    if (b->tfull) return fused_stage<F, false>(b, dt, stage, qin, qout);
-    int rc = launch_weno_lists<F>(b, qin);
-    if (rc) return rc;
   }
+  // first stage of a folded step: the flux pass also forms dt
+  double* lam = c.viscous && b->dt_fold && stage == 1 ? b->lam_part : nullptr;
+  if (c.viscous && (rc = viscous_part<F>(b, lam, 1))) return rc;
+  if (side) {
+    CBNS_HIP(b, hipStreamWaitEvent(b->stream, b->ev_join, 0));
+    if ((rc = launch_face_lists(b))) return rc;
+  }
+  // more troubled faces than the lists hold: WENO at every face
+  const bool hyb = hyb_form && !b->tfull;
+  if (hyb && (rc = launch_weno_lists<F>(b, qin))) return rc;
   const long S = c.S;
-  const uint8_t* fl = HYB ? b->flags : nullptr;
+  const uint8_t* fl = hyb ? b->flags : nullptr;
   if (!FUSE_XI) {
     Box box;
     for (int a = 0; a < 3; a++) {
@@ -574,63 +629,40 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
     box.n[0] = c.IM - 1 + (periodic(c, 0) ? 2 : 0);  // faces -1..N-1 or 0..N-2
     ProfScope ps(b, CBNS_K_XFACES);
     const TroubledFaces tx{b->tslot, b->tflux, b->ttotal};
-    k_xfaces<F, HYB><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fl, tx, fx);
+    if (hyb)
+      k_xfaces<F, true><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fl, tx, fx);
+    else
+      k_xfaces<F, false><<<box_grid(box), box_block(), 0, b->stream>>>(c, box, qin, b->met, fl, tx, fx);
     CBNS_HIP(b, hipGetLastError());
   }
   const int pmask = stage_prim_mask(b, stage);
   StageArgs sa{dt, b->dt_fold ? b->red + 2 : nullptr, stage, b->prim, b->Q, qin, qout, fx,
-               b->rhs, nullptr, HYB ? fl + S : nullptr, HYB ? fl : nullptr, pmask,
-               TroubledFaces{HYB ? b->tslot + S : nullptr, b->tflux, b->ttotal},
-               TroubledFaces{HYB ? b->tslot : nullptr, b->tflux, b->ttotal}};
-  const int ni = upd_hi(c, 0) - upd_lo(c, 0) + 1;
-  {
-    const int nt = upd_hi(c, 2) - upd_lo(c, 2) + 1;
-    const int nm = upd_hi(c, 1) - upd_lo(c, 1) + 1;
-    const int per_wave = march_width<1, F>();
-    dim3 grid((ni + per_wave - 1) / per_wave, (nt + MARCH_Y - 1) / MARCH_Y,
-              (nm + CHUNK - 1) / CHUNK);
-    ProfScope ps(b, CBNS_K_MARCH_ETA);
-    k_march<1, F, HYB><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
-    CBNS_HIP(b, hipGetLastError());
-  }
+               b->rhs, nullptr, hyb ? fl + S : nullptr, hyb ? fl : nullptr, pmask,
+               TroubledFaces{hyb ? b->tslot + S : nullptr, b->tflux, b->ttotal},
+               TroubledFaces{hyb ? b->tslot : nullptr, b->tflux, b->ttotal}};
+  if ((rc = hyb ? launch_marches_eta<F, true>(b, sa) : launch_marches_eta<F, false>(b, sa)))
+    return rc;
   if (c.viscous) {
-    // first stage of a folded step: the flux pass also forms dt
-    double* lam = b->dt_fold && stage == 1 ? b->lam_part : nullptr;
-    int rc = sizeof(F) == sizeof(double) ? launch_viscous(b, b->vtmp, b->rhs, 1, lam)
-                                         : launch_viscous_f32(b, b->vtmp, b->rhs, lam);
-    if (rc) return rc;
+    if ((rc = viscous_part<F>(b, nullptr, 2))) return rc;
     if (lam) {
       ProfScope ps(b, CBNS_K_DT);
       if ((rc = launch_dt_from_partials(b, lam, (int)visc_flux_blocks(c), b->dt_cap, b->red + 2)))
         return rc;
     }
   }
-  {
-    const int nt = upd_hi(c, 1) - upd_lo(c, 1) + 1;
-    const int nm = upd_hi(c, 2) - upd_lo(c, 2) + 1;
-    const int per_wave = march_width<2, F>();
-    dim3 grid((ni + per_wave - 1) / per_wave, (nt + MARCH_Y - 1) / MARCH_Y,
-              (nm + CHUNK - 1) / CHUNK);
-    ProfScope ps(b, CBNS_K_MARCH_ZETA);
-    if (HYB) {
-      sa.flags = fl + 2 * S;
-      sa.tf.slot = b->tslot + 2 * S;
-    }
-    k_march<2, F, HYB><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
-    CBNS_HIP(b, hipGetLastError());
+  if (hyb) {
+    sa.flags = fl + 2 * S;
+    sa.tf.slot = b->tslot + 2 * S;
   }
-  return CBNS_OK;
+  return hyb ? launch_marches_zeta<F, true>(b, sa) : launch_marches_zeta<F, false>(b, sa);
 }
 
 bool fused_stage_uses_face_flux() { return !FUSE_XI; }
 
 int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout) {
-  switch (b->form) {
-    case CBNS_FORM_FP32: return fused_stage<float, false>(b, dt, stage, qin, qout);
-    case CBNS_FORM_HYBRID: return fused_stage<double, true>(b, dt, stage, qin, qout);
-    case CBNS_FORM_FP32 | CBNS_FORM_HYBRID: return fused_stage<float, true>(b, dt, stage, qin, qout);
-    default: return fused_stage<double, false>(b, dt, stage, qin, qout);
-  }
+  const bool hyb = (b->form & CBNS_FORM_HYBRID) != 0;
+  return (b->form & CBNS_FORM_FP32) ? fused_stage<float>(b, dt, stage, qin, qout, hyb)
+                                    : fused_stage<double>(b, dt, stage, qin, qout, hyb);
 }
 
 }  // namespace cbns

@@ -369,23 +369,23 @@ int launch_face_samples(cbns_block* b, int axis, const double* Q, float* feat, f
   return CBNS_OK;
 }
 
-int launch_indicator(cbns_block* b, const double* Q) {
+int launch_indicator(cbns_block* b, const double* Q, hipStream_t st) {
   const MlpDev w{reinterpret_cast<const bf16x4*>(b->ind_pack),
                  reinterpret_cast<const bf16x4*>(b->ind_pack) + 256, nullptr, b->ind_P};
   unsigned* cnt = b->ind_count_on ? b->ind_count : nullptr;
   if (cnt) {
-    k_zero_counts<<<1, 64, 0, b->stream>>>(cnt);
+    k_zero_counts<<<1, 64, 0, st>>>(cnt);
     CBNS_HIP(b, hipGetLastError());
   }
-  ProfScope ps(b, CBNS_K_INDICATOR);
+  ProfScope ps(b, CBNS_K_INDICATOR, st);
   for (int a = 0; a < 3; a++) {
     const Box box = face_box(b->c, a, 1);
     if (box.count() == 0) continue;
     uint8_t* fl = b->flags + (long)a * b->c.S;
     switch (a) {
-      case 0: k_indicator<0><<<box_grid(box), box_block(), 0, b->stream>>>(b->c, box, Q, b->met, w, b->ind_thresh, fl, cnt ? cnt + 0 : nullptr); break;
-      case 1: k_indicator<1><<<box_grid(box), box_block(), 0, b->stream>>>(b->c, box, Q, b->met, w, b->ind_thresh, fl, cnt ? cnt + 1 : nullptr); break;
-      default: k_indicator<2><<<box_grid(box), box_block(), 0, b->stream>>>(b->c, box, Q, b->met, w, b->ind_thresh, fl, cnt ? cnt + 2 : nullptr); break;
+      case 0: k_indicator<0><<<box_grid(box), box_block(), 0, st>>>(b->c, box, Q, b->met, w, b->ind_thresh, fl, cnt ? cnt + 0 : nullptr); break;
+      case 1: k_indicator<1><<<box_grid(box), box_block(), 0, st>>>(b->c, box, Q, b->met, w, b->ind_thresh, fl, cnt ? cnt + 1 : nullptr); break;
+      default: k_indicator<2><<<box_grid(box), box_block(), 0, st>>>(b->c, box, Q, b->met, w, b->ind_thresh, fl, cnt ? cnt + 2 : nullptr); break;
     }
     CBNS_HIP(b, hipGetLastError());
   }

@@ -381,9 +381,9 @@ long visc_flux_blocks(const Consts& c) { return (long)tile_march_blocks(visc_flu
 // lam_part (visc_flux_blocks entries) or null: the flux pass also forms the
 // time step's per-workgroup spectral-radius maxima.
 template <class T>
-int viscous(cbns_block* b, T* vf, double* out, int mode, double* lam_part) {
+int viscous(cbns_block* b, T* vf, double* out, int mode, double* lam_part, int parts) {
   const Consts& c = b->c;
-  {
+  if (parts & 1) {
     ProfScope ps(b, CBNS_K_VISC_FLUX);
     const TileMarch tm = visc_flux_tiles(c);
     const dim3 grid(tile_march_blocks(tm)), block(BOX_X, FROWS);
@@ -395,6 +395,7 @@ int viscous(cbns_block* b, T* vf, double* out, int mode, double* lam_part) {
                                                              nullptr);
     CBNS_HIP(b, hipGetLastError());
   }
+  if (!(parts & 2)) return CBNS_OK;
   Box ub;
   for (int a = 0; a < 3; a++) {
     ub.lo[a] = upd_lo(c, a);
@@ -407,13 +408,13 @@ int viscous(cbns_block* b, T* vf, double* out, int mode, double* lam_part) {
   return CBNS_OK;
 }
 
-int launch_viscous(cbns_block* b, double* vf, double* out, int mode, double* lam_part) {
-  return viscous<double>(b, vf, out, mode, lam_part);
+int launch_viscous(cbns_block* b, double* vf, double* out, int mode, double* lam_part, int parts) {
+  return viscous<double>(b, vf, out, mode, lam_part, parts);
 }
 
 // precision form: the same scratch holds 12*S floats
-int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs, double* lam_part) {
-  return viscous<float>(b, reinterpret_cast<float*>(scratch), rhs, 1, lam_part);
+int launch_viscous_f32(cbns_block* b, double* scratch, double* rhs, double* lam_part, int parts) {
+  return viscous<float>(b, reinterpret_cast<float*>(scratch), rhs, 1, lam_part, parts);
 }
 
 }  // namespace cbns
