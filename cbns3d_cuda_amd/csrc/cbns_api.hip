@@ -234,10 +234,6 @@ void cbns_destroy(cbns_block* b) {
     (void)hipEventDestroy(ev.first);
     (void)hipEventDestroy(ev.second);
   }
-  if (b->stream2) (void)hipStreamSynchronize(b->stream2);
-  if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
-  if (b->ev_join) (void)hipEventDestroy(b->ev_join);
-  if (b->stream2) (void)hipStreamDestroy(b->stream2);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
@@ -526,11 +522,6 @@ int cbns_set_indicator(cbns_block* b, const float* params, float threshold, int3
   if (!b->flags) {
     CBNS_HIP(b, hipMalloc((void**)&b->flags, 3 * (size_t)b->c.S));
     CBNS_HIP(b, hipMemsetAsync(b->flags, 0, 3 * (size_t)b->c.S, b->stream));
-  }
-  if (!b->stream2) {
-    CBNS_HIP(b, hipStreamCreate(&b->stream2));
-    CBNS_HIP(b, hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming));
-    CBNS_HIP(b, hipEventCreateWithFlags(&b->ev_join, hipEventDisableTiming));
   }
   // troubled-face lists: a third of each axis' faces, or all of them
   long cap[3], total = 0;

@@ -253,9 +253,6 @@ struct cbns_block {
   cbns::Consts c{};
   int device = 0;
   hipStream_t stream = nullptr;
-  // second stream (hybrid form): the indicator beside the viscous flux pass
-  hipStream_t stream2 = nullptr;
-  hipEvent_t ev_fork = nullptr, ev_join = nullptr;
   bool have_mesh = false;
   int form = CBNS_FORM_FP64;
   // device arrays
@@ -369,7 +366,7 @@ bool fused_stage_uses_face_flux();
 int launch_fill(hipStream_t s, double* p, long n, double v);
 long face_count(const cbns_block* b, int axis);
 int launch_face_samples(cbns_block* b, int axis, const double* Q, float* feat, float* label);
-int launch_indicator(cbns_block* b, const double* Q, hipStream_t s);
+int launch_indicator(cbns_block* b, const double* Q);
 // after the flags change: list the troubled faces per axis and read the
 // counts back (sets tfull when a list overflows its capacity)
 int launch_face_lists(cbns_block* b);

@@ -34,6 +34,12 @@ namespace {
 
 constexpr int MARCH_X = 64;  // lanes along i
 constexpr int MARCH_Y = 4;   // lines per workgroup along the other axis
+#ifndef ZETA_ROWS
+#define ZETA_ROWS 4  // the zeta march's lines along j per workgroup
+#endif
+// workgroup size of a march (the eta march: MARCH_Y wavefronts on one line)
+template <int AX>
+constexpr int march_rows() { return AX == 2 ? ZETA_ROWS : MARCH_Y; }
 #ifndef MARCH_CHUNK
 #define MARCH_CHUNK 32
 #endif
@@ -145,7 +151,7 @@ struct StageArgs {
 
 // AX = 1: march along eta, rhs = dE + dF.  AX = 2: march along zeta, finish.
 template <int AX, class F, bool HYB>
-__global__ void __launch_bounds__(MARCH_X * MARCH_Y)
+__global__ void __launch_bounds__(MARCH_X * march_rows<AX>())
 __attribute__((amdgpu_waves_per_eu(HYB ? WPE_H : WPE_M, HYB ? WPE_H : WPE_M)))
 k_march(Consts c, StageArgs sa,
                                                             const double* __restrict__ met) {
@@ -164,8 +170,15 @@ k_march(Consts c, StageArgs sa,
   // eta march by arithmetic).
   constexpr bool XSH = XI && AX == 2;
   constexpr int XL = XSH ? 3 : (XI ? 1 : 0);  // lanes left of the first updated node
-  const int i = upd_lo(c, 0) - XL + blockIdx.x * march_width<AX, F>() + lane;
-  const int t = upd_lo(c, ax_t) + blockIdx.y * MARCH_Y + threadIdx.y;
+  // eta march: the workgroup's wavefronts take adjacent tiles of one line
+  // along i, so each step along eta reads whole rows of the fields (-9 % for
+  // the FP64 eta march, -12 % FP32).  zeta march: wavefronts on adjacent
+  // lines (its 58-node tiles would leave idle wavefronts on a line)
+  constexpr bool ROWS = AX == 1;
+  const int i = upd_lo(c, 0) - XL +
+                (ROWS ? blockIdx.x * MARCH_Y + threadIdx.y : blockIdx.x) * march_width<AX, F>() +
+                lane;
+  const int t = upd_lo(c, ax_t) + (ROWS ? blockIdx.y : blockIdx.y * march_rows<AX>() + threadIdx.y);
   if (i > upd_hi(c, 0) + (XSH ? 3 : 0) || t > upd_hi(c, ax_t)) return;
   const bool own = lane >= XL && lane < XL + march_width<AX, F>() && i <= upd_hi(c, 0);
   const int mlo = upd_lo(c, AX), mhi = upd_hi(c, AX);
@@ -548,7 +561,8 @@ int launch_marches_eta(cbns_block* b, StageArgs& sa) {
   const int nt = upd_hi(c, 2) - upd_lo(c, 2) + 1;
   const int nm = upd_hi(c, 1) - upd_lo(c, 1) + 1;
   const int per_wave = march_width<1, F>();
-  dim3 grid((ni + per_wave - 1) / per_wave, (nt + MARCH_Y - 1) / MARCH_Y, (nm + CHUNK - 1) / CHUNK);
+  const int nwx = (ni + per_wave - 1) / per_wave;
+  dim3 grid((nwx + MARCH_Y - 1) / MARCH_Y, nt, (nm + CHUNK - 1) / CHUNK);
   ProfScope ps(b, CBNS_K_MARCH_ETA);
   k_march<1, F, HYB><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
   CBNS_HIP(b, hipGetLastError());
@@ -562,9 +576,10 @@ int launch_marches_zeta(cbns_block* b, StageArgs& sa) {
   const int nt = upd_hi(c, 1) - upd_lo(c, 1) + 1;
   const int nm = upd_hi(c, 2) - upd_lo(c, 2) + 1;
   const int per_wave = march_width<2, F>();
-  dim3 grid((ni + per_wave - 1) / per_wave, (nt + MARCH_Y - 1) / MARCH_Y, (nm + CHUNK - 1) / CHUNK);
+  constexpr int R = march_rows<2>();
+  dim3 grid((ni + per_wave - 1) / per_wave, (nt + R - 1) / R, (nm + CHUNK - 1) / CHUNK);
   ProfScope ps(b, CBNS_K_MARCH_ZETA);
-  k_march<2, F, HYB><<<grid, dim3(MARCH_X, MARCH_Y), 0, b->stream>>>(c, sa, b->met);
+  k_march<2, F, HYB><<<grid, dim3(MARCH_X, R), 0, b->stream>>>(c, sa, b->met);
   CBNS_HIP(b, hipGetLastError());
   return CBNS_OK;
 }
@@ -575,34 +590,22 @@ int viscous_part(cbns_block* b, double* lam, int parts) {
                                      : launch_viscous_f32(b, b->vtmp, b->rhs, lam, parts);
 }
 
-// One fused stage.  Order: [indicator (hybrid, when due)] + viscous flux
-// pass, eta march, viscous divergence, zeta march.  The flux pass reads only
-// the stage input's primitives, so it runs before the eta march; on a viscous
-// hybrid stage whose indicator is due, the indicator (bound by its
-// arithmetic) runs on the block's second stream beside the flux pass (bound
-// by memory), and the face lists wait for both.
+// One fused stage: [indicator and face lists (hybrid, when due)], viscous
+// flux pass, eta march, viscous divergence, zeta march.  The flux pass reads
+// only the stage input's primitives, so it may run before the eta march.
 template <class F>
 int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout,
                 bool hyb_form) {
   const Consts& c = b->c;
   double* fx = b->flux;  // first 5 face arrays
   int rc = CBNS_OK;
-  bool side = false;
   if (hyb_form) {
     // once per step on the stage-1 input (or every ind_every steps, or at
     // every stage)
     const bool due = !b->flags_valid || b->ind_per_stage ||
                      (stage == 1 && b->ind_age >= b->ind_every);
     if (due) {
-      if (c.viscous && b->stream2) {
-        CBNS_HIP(b, hipEventRecord(b->ev_fork, b->stream));
-        CBNS_HIP(b, hipStreamWaitEvent(b->stream2, b->ev_fork, 0));
-        if ((rc = launch_indicator(b, qin, b->stream2))) return rc;
-        CBNS_HIP(b, hipEventRecord(b->ev_join, b->stream2));
-        side = true;
-      } else if ((rc = launch_indicator(b, qin, b->stream)) || (rc = launch_face_lists(b))) {
-        return rc;
-      }
+      if ((rc = launch_indicator(b, qin)) || (rc = launch_face_lists(b))) return rc;
       b->ind_age = 0;
     }
     if (stage == 1) b->ind_age++;
@@ -610,10 +613,6 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
   // first stage of a folded step: the flux pass also forms dt
   double* lam = c.viscous && b->dt_fold && stage == 1 ? b->lam_part : nullptr;
   if (c.viscous && (rc = viscous_part<F>(b, lam, 1))) return rc;
-  if (side) {
-    CBNS_HIP(b, hipStreamWaitEvent(b->stream, b->ev_join, 0));
-    if ((rc = launch_face_lists(b))) return rc;
-  }
   // more troubled faces than the lists hold: WENO at every face
   const bool hyb = hyb_form && !b->tfull;
   if (hyb && (rc = launch_weno_lists<F>(b, qin))) return rc;

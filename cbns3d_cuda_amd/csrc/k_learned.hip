@@ -369,7 +369,8 @@ int launch_face_samples(cbns_block* b, int axis, const double* Q, float* feat, f
   return CBNS_OK;
 }
 
-int launch_indicator(cbns_block* b, const double* Q, hipStream_t st) {
+int launch_indicator(cbns_block* b, const double* Q) {
+  hipStream_t st = b->stream;
   const MlpDev w{reinterpret_cast<const bf16x4*>(b->ind_pack),
                  reinterpret_cast<const bf16x4*>(b->ind_pack) + 256, nullptr, b->ind_P};
   unsigned* cnt = b->ind_count_on ? b->ind_count : nullptr;
