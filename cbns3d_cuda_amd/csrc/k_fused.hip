@@ -14,8 +14,10 @@
 //                dE + dF with the xi-faces here.
 //                (FUSE_XI=0: a separate k_xfaces pass writes the xi-face
 //                fluxes to HBM and the march reads them back.)
-//   [viscous]    k_visc_fluxes + k_visc_div subtract the viscous divergence
-//                from rhs (k_viscous.hip)
+//   [viscous]    k_visc_fluxes (k_viscous.hip) writes the viscous node
+//                fluxes before the marches; each march subtracts their
+//                4th-order face values from its face fluxes (visc_face), so
+//                the divergence needs no pass of its own
 //   k_march<2>   the same march along zeta; adds dG, divides by J and
 //                applies the Shu-Osher stage combination, writing the stage
 //                output and its primitives (the boundary pass then only
@@ -85,6 +87,20 @@ struct TroubledFaces {
   long stride;
 };
 
+// Viscous terms in face form.  The reference's viscous contribution is the
+// 4th-order central difference (1/12, -2/3, 0, 2/3, -1/12) of each node flux
+// along its axis (src/viscous_contribution_kernel.cu:44-67); that difference
+// at node m equals h(m+1/2) - h(m-1/2) with the face value
+// h(m+1/2) = 7/12 (v[m] + v[m+1]) - 1/12 (v[m-1] + v[m+2]), so the marches
+// subtract h from the inviscid face fluxes they already difference: the eta
+// march h of Fv, the zeta march h of Gv and, at its xi-faces, h of Ev.  The
+// differences are taken on the reference's lines (offset by -1 in both
+// transverse indices; 0 for the centred divergence).
+template <class T>
+__device__ __forceinline__ T visc_face(T vm1, T v0, T v1, T v2) {
+  return T(7.0 / 12.0) * (v0 + v1) - T(1.0 / 12.0) * (vm1 + v2);
+}
+
 // Face flux of the stage's form at face `face` (node offset).  HYB: the
 // linear-weight reconstruction, replaced by the face's characteristic-WENO
 // flux from tf where the face is troubled (its flag or a neighbour's along
@@ -141,7 +157,7 @@ struct StageArgs {
   double* Qout;         // stage output
   const double* fx;     // xi-face fluxes (AX == 1)
   double* rhs;          // dE + dF (+ viscous)
-  const double* dfv;    // viscous divergence or nullptr
+  const void* vf;       // viscous node fluxes Ev, Fv, Gv (12 fields of the face type) or null
   const uint8_t* flags; // troubled faces of the march axis (HYB)
   const uint8_t* flags_x;  // troubled xi-faces (HYB, fused xi-faces)
   int pmask;            // AX == 2: primitives to write (bit n: field n of rho, u, v, w, p, T)
@@ -235,6 +251,39 @@ k_march(Consts c, StageArgs sa,
     stage_face<F, HYB>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), t0 | t1 | t2,
                        sa.tf, base + (m0 - 1) * st, fprev);
   }
+  // viscous node flux of the march axis (Fv or Gv) on its offset line,
+  // nodes m-1, m, m+1 of the face m+1/2 (visc_face); Ev of the xi-faces on
+  // the zeta march's offset line
+  const bool VIS = sa.vf != nullptr;
+  const int vsh = c.visc_centered ? 0 : -1;
+  const F* __restrict__ ve = static_cast<const F*>(sa.vf);
+  const F* __restrict__ va = VIS ? ve + (long)(4 * AX) * S : nullptr;
+  const int vbase = AX == 1 ? idx(c, i + vsh, 0, t + vsh) : idx(c, i + vsh, t + vsh, 0);
+  const int ebase = idx(c, i, t + vsh, vsh);  // XSH: Ev of node m at ebase + m*st
+  // The window (nodes m-1, m, m+1) is held in registers, or, in the FP64
+  // zeta march, whose registers the xi-faces take, in a per-thread LDS ring
+  // (slot vr holds node m-1, the next two slots the others; no thread reads
+  // another's entries, so no barrier).
+  constexpr bool VRING = AX == 2 && XSH && sizeof(F) == sizeof(double) && !HYB;
+  constexpr int RR = VRING ? march_rows<AX>() : 1;
+  __shared__ F vring[VRING ? 3 : 1][4][RR][VRING ? MARCH_X : 1];
+  const int ty = VRING ? threadIdx.y : 0, tl = VRING ? lane : 0;
+  F vw[3][4];
+  int vr = 0;
+  if (VIS) {
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      F w[4];
+#pragma unroll
+      for (int s = 0; s < 4; s++) w[s] = va[e * S + vbase + (m0 - 2 + s) * st];
+      fprev[e + 1] -= (double)visc_face(w[0], w[1], w[2], w[3]);
+#pragma unroll
+      for (int s = 0; s < 3; s++) {
+        if constexpr (VRING) vring[s][e][ty][tl] = w[s + 1];
+        else vw[s][e] = w[s + 1];
+      }
+    }
+  }
   // eta march: the next node's state and the next face's metrics are
   // fetched one iteration ahead (nodes up to mhi + 4 lie in the ghosted
   // array): -1.6 % for the FP64 eta march; the hybrid's eta march, bound by
@@ -259,6 +308,14 @@ k_march(Consts c, StageArgs sa,
       load_q(Q, S, o + 4 * st, qpf);
     } else {
       load_q(Q, S, o + 3 * st, q[5]);
+    }
+    F vn[4], ev[4];  // viscous flux of node m+2 along the axis; Ev of node m (XSH)
+    if (VIS) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        vn[e] = va[e * S + vbase + (m + 2) * st];
+        if constexpr (XSH) ev[e] = ve[e * S + ebase + m * st];
+      }
     }
     if constexpr (DC) {
 #pragma unroll
@@ -372,6 +429,16 @@ k_march(Consts c, StageArgs sa,
       } else {
         stage_face<F, HYB>(g, qx, nxx, nxy, nxz, tx, sa.tf_x, o, fxr);
       }
+      if constexpr (XSH) {
+        if (VIS) {
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            const F v = ev[e];
+            fxr[e + 1] -= (double)visc_face(__shfl(v, lane - 1), v, __shfl(v, lane + 1),
+                                            __shfl(v, lane + 2));
+          }
+        }
+      }
     }
     if (HYB) {
       t0 = t1;
@@ -386,6 +453,23 @@ k_march(Consts c, StageArgs sa,
     } else {
       stage_face<F, HYB>(g, q, 0.5 * (ax + bx), 0.5 * (ay + by), 0.5 * (az + bz), t0 | t1 | t2,
                          sa.tf, o, f);
+    }
+    if (VIS) {
+      const int s0 = vr, s1 = vr == 2 ? 0 : vr + 1, s2 = vr == 0 ? 2 : vr - 1;
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        if constexpr (VRING) {
+          f[e + 1] -= (double)visc_face(vring[s0][e][ty][tl], vring[s1][e][ty][tl],
+                                        vring[s2][e][ty][tl], vn[e]);
+          vring[s0][e][ty][tl] = vn[e];  // node m+2 takes node m-1's slot
+        } else {
+          f[e + 1] -= (double)visc_face(vw[0][e], vw[1][e], vw[2][e], vn[e]);
+          vw[0][e] = vw[1][e];
+          vw[1][e] = vw[2][e];
+          vw[2][e] = vn[e];
+        }
+      }
+      vr = s1;
     }
     double dfx[5];  // xi-face difference (XI)
     if (XI) {
@@ -419,7 +503,6 @@ k_march(Consts c, StageArgs sa,
       for (int n = 0; n < 5; n++) {
         double df = rin[n] + (f[n] - fprev[n]);
         if (XI) df += dfx[n];
-        if (n > 0 && sa.dfv) df -= sa.dfv[(n - 1) * S + o];
         df *= inv_j;
         const double qin = q[2][n];
         double out;
@@ -449,35 +532,69 @@ k_march(Consts c, StageArgs sa,
 // The troubled faces of axis AX in the flux kernels' face range fb: face m
 // is troubled when its flag or a neighbour's along AX is set (the same
 // one-cell dilation the marches apply).  Each is appended to list (node
-// offset) and its slot (slot_base + position) recorded in slot[face]; a
-// wavefront reserves its positions with one atomic.  Positions past cap are
-// counted but not written (the host then runs full WENO).
+// offset) and its slot (slot_base + position) recorded in slot[face].  A
+// thread takes LIST_PLANES faces along k and a workgroup (64 x 4 threads)
+// reserves the positions of all its faces with one atomic, so the counter
+// sees one request per 4096 faces (one per wavefront and plane measured
+// 70 ms per step at 512^3 with 60 % of the faces troubled: same-address
+// atomics serialise).  Positions past cap are not written (the host then
+// runs full WENO); once the count has passed cap, workgroups stop adding.
+constexpr int LIST_PLANES = 16;
+
 template <int AX>
 __global__ void __launch_bounds__(256) k_face_list(Consts c, Box fb,
                                                    const uint8_t* __restrict__ fl,
                                                    int* __restrict__ slot, int* __restrict__ list,
                                                    unsigned* __restrict__ count, unsigned cap,
                                                    int slot_base) {
-  int i = 0, j = 0, k = 0;
-  const bool valid = box_node3(fb, i, j, k);
-  int a = 0;
-  bool tr = false;
-  if (valid) {
-    a = idx(c, i, j, k);
-    const int st = stride(c, AX);
-    tr = (fl[a - st] | fl[a] | fl[a + st]) != 0;
+  const int lane = threadIdx.x, w = threadIdx.y;  // 64 x 4: one wavefront per row
+  const int i = fb.lo[0] + (int)blockIdx.x * 64 + lane;
+  const int j = fb.lo[1] + (int)blockIdx.y * 4 + w;
+  const int k0 = fb.lo[2] + (int)blockIdx.z * LIST_PLANES;
+  const int np = min(LIST_PLANES, fb.lo[2] + fb.n[2] - k0);
+  const bool line = i < fb.lo[0] + fb.n[0] && j < fb.lo[1] + fb.n[1];
+  const int st = stride(c, AX), sz = c.PXY;
+  const int a0 = idx(c, i, j, k0);
+  unsigned mine = 0;       // bit p: face (i, j, k0 + p) is troubled
+  unsigned long long bal[LIST_PLANES];
+  unsigned nwave = 0;
+#pragma unroll
+  for (int p = 0; p < LIST_PLANES; p++) {
+    bool tr = false;
+    if (line && p < np) {
+      const int a = a0 + p * sz;
+      tr = (fl[a - st] | fl[a] | fl[a + st]) != 0;
+    }
+    bal[p] = __ballot(tr);
+    mine |= (unsigned)tr << p;
+    nwave += (unsigned)__popcll(bal[p]);
   }
-  const unsigned long long bal = __ballot(tr);
-  if (bal == 0ull) return;  // the whole wavefront
-  const int lane = threadIdx.x;  // one wavefront per row of the 64 x 4 block
-  const int first = __ffsll((long long)bal) - 1;
-  unsigned pos0 = 0;
-  if (lane == first) pos0 = atomicAdd(count, (unsigned)__popcll(bal));
-  pos0 = __shfl(pos0, first);
-  const unsigned pos = pos0 + (unsigned)__popcll(bal & ((1ull << lane) - 1ull));
-  if (tr && pos < cap) {
-    list[pos] = a;
-    slot[a] = slot_base + (int)pos;
+  __shared__ unsigned wn[4], wbase;
+  if (lane == 0) wn[w] = nwave;
+  __syncthreads();
+  if (lane == 0 && w == 0) {
+    const unsigned tot = wn[0] + wn[1] + wn[2] + wn[3];
+    // past cap already (the count only grows): list nothing, positions from cap
+    unsigned b = cap;
+    if (tot && __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) <= cap)
+      b = atomicAdd(count, tot);
+    wbase = b;
+  }
+  __syncthreads();
+  unsigned pos = wbase;
+  for (int q = 0; q < w; q++) pos += wn[q];
+  const unsigned long long below = (1ull << lane) - 1ull;
+#pragma unroll
+  for (int p = 0; p < LIST_PLANES; p++) {
+    if ((mine >> p) & 1u) {
+      const unsigned at = pos + (unsigned)__popcll(bal[p] & below);
+      if (at < cap) {
+        const int a = a0 + p * sz;
+        list[at] = a;
+        slot[a] = slot_base + (int)at;
+      }
+    }
+    pos += (unsigned)__popcll(bal[p]);
   }
 }
 
@@ -511,7 +628,8 @@ __global__ void __launch_bounds__(256) k_weno_list(Consts c, const int* __restri
 
 template <int AX>
 void launch_face_list_axis(cbns_block* b, const Box& fb) {
-  k_face_list<AX><<<box_grid(fb), box_block(), 0, b->stream>>>(
+  const dim3 grid((fb.n[0] + 63) / 64, (fb.n[1] + 3) / 4, (fb.n[2] + LIST_PLANES - 1) / LIST_PLANES);
+  k_face_list<AX><<<grid, dim3(64, 4), 0, b->stream>>>(
       b->c, fb, b->flags + (long)AX * b->c.S, b->tslot + (long)AX * b->c.S, b->tlist + b->toff[AX],
       b->tcount + AX, (unsigned)b->tcap[AX], (int)b->toff[AX]);
 }
@@ -590,9 +708,18 @@ int viscous_part(cbns_block* b, double* lam, int parts) {
                                      : launch_viscous_f32(b, b->vtmp, b->rhs, lam, parts);
 }
 
+// The marches take the viscous divergence in face form (visc_face) when the
+// zeta march holds the xi-faces; otherwise k_visc_div subtracts it from the
+// right-hand side between the marches.
+constexpr int TFULL_EVERY = 8;  // steps between indicator passes while a list overflows
+
+template <class F>
+constexpr bool visc_in_marches() { return FUSE_XI && xi_axis<F>() == 2; }
+
 // One fused stage: [indicator and face lists (hybrid, when due)], viscous
-// flux pass, eta march, viscous divergence, zeta march.  The flux pass reads
-// only the stage input's primitives, so it may run before the eta march.
+// flux pass, eta march, zeta march.  The flux pass reads only the stage
+// input's primitives, so it may run before the eta march; the marches
+// difference its node fluxes along with the inviscid face fluxes.
 template <class F>
 int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout,
                 bool hyb_form) {
@@ -601,9 +728,12 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
   int rc = CBNS_OK;
   if (hyb_form) {
     // once per step on the stage-1 input (or every ind_every steps, or at
-    // every stage)
-    const bool due = !b->flags_valid || b->ind_per_stage ||
-                     (stage == 1 && b->ind_age >= b->ind_every);
+    // every stage).  While a list has overflowed, every face takes WENO and
+    // the flags go unused: the pass then runs every TFULL_EVERY steps, to
+    // see whether the lists fit again.
+    const int every = b->tfull ? std::max(b->ind_every, TFULL_EVERY) : b->ind_every;
+    const bool due = !b->flags_valid || (b->ind_per_stage && !b->tfull) ||
+                     (stage == 1 && b->ind_age >= every);
     if (due) {
       if ((rc = launch_indicator(b, qin)) || (rc = launch_face_lists(b))) return rc;
       b->ind_age = 0;
@@ -636,13 +766,14 @@ int fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* 
   }
   const int pmask = stage_prim_mask(b, stage);
   StageArgs sa{dt, b->dt_fold ? b->red + 2 : nullptr, stage, b->prim, b->Q, qin, qout, fx,
-               b->rhs, nullptr, hyb ? fl + S : nullptr, hyb ? fl : nullptr, pmask,
+               b->rhs, c.viscous && visc_in_marches<F>() ? b->vtmp : nullptr,
+               hyb ? fl + S : nullptr, hyb ? fl : nullptr, pmask,
                TroubledFaces{hyb ? b->tslot + S : nullptr, b->tflux, b->ttotal},
                TroubledFaces{hyb ? b->tslot : nullptr, b->tflux, b->ttotal}};
   if ((rc = hyb ? launch_marches_eta<F, true>(b, sa) : launch_marches_eta<F, false>(b, sa)))
     return rc;
   if (c.viscous) {
-    if ((rc = viscous_part<F>(b, nullptr, 2))) return rc;
+    if (!visc_in_marches<F>() && (rc = viscous_part<F>(b, nullptr, 2))) return rc;
     if (lam) {
       ProfScope ps(b, CBNS_K_DT);
       if ((rc = launch_dt_from_partials(b, lam, (int)visc_flux_blocks(c), b->dt_cap, b->red + 2)))

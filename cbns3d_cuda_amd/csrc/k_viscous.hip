@@ -12,7 +12,9 @@
 //   k_visc_div     4th-order central divergence of Ev,Fv,Gv at the update
 //                  nodes, at the reference's transverse offset (-1) or
 //                  centred; written to dfv (reference decomposition) or
-//                  subtracted from the fused stage's right-hand side.
+//                  subtracted from a right-hand side.  The fused stage
+//                  does without it: its marches subtract the fluxes' face
+//                  values from their face fluxes (k_fused.hip, visc_face).
 // Both walk along k in chunks (TileMarch): the five zeta-planes a 4th-order
 // stencil needs (u,v,w,T in the flux pass, Gv in the divergence) stay in a
 // register window, so each plane value is loaded once per chunk instead of

@@ -207,7 +207,8 @@ int cbns_face_samples(cbns_block* blk, int32_t axis, int32_t buf, float* d_feat,
  * are listed when the flags change and their WENO fluxes formed in one pass
  * per stage; lists hold a third of an axis' faces, and a step whose
  * troubled faces exceed that runs WENO at every face (the plain form's
- * marches) -- bit 3: lists that hold every face (never that fallback). */
+ * marches; the indicator then runs only every 8 steps, until the lists fit
+ * again) -- bit 3: lists that hold every face (never that fallback). */
 int cbns_set_indicator(cbns_block* blk, const float* params, float threshold, int32_t options);
 /* troubled faces (before dilation) per axis found by the last indicator pass
  * among the faces the flux kernels compute (cbns_face_count); needs options
