@@ -37,9 +37,11 @@ PEAK_HBM_GBS = 8000.0
 FLOPS_PER_FACE = 1580.0
 # Algorithmic HBM bytes of the memory-bound kernels (compulsory traffic, each
 # value moved once): viscous fluxes read u,v,w,T and the 10 metrics and write
-# Ev,Fv,Gv (12) per node of the viscous range (N+4)^3; the divergence reads
-# the 12 fluxes and read-modify-writes the 4 momentum/energy rhs components
-# per updated node; dt reads 6 primitives and 9 metrics per node.
+# Ev,Fv,Gv (12) per node of the viscous range (N+4)^3; a divergence pass
+# (reference-decomposed calls only: the fused stage's marches take the
+# divergence) reads the 12 fluxes and read-modify-writes the 4
+# momentum/energy rhs components per updated node; dt reads 6 primitives and
+# 9 metrics per node.
 VISC_FLUX_BYTES_PER_NODE = (4 + 10 + 12) * 8
 VISC_DIV_BYTES_PER_NODE = (12 + 2 * 4) * 8
 DT_BYTES_PER_NODE = (6 + 9) * 8

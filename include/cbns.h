@@ -163,10 +163,10 @@ int64_t cbns_field_size(const cbns_block* blk);
  * reset and reset=1 clears it. */
 enum {
   CBNS_K_XFACES = 0,     /* xi-face fluxes                          */
-  CBNS_K_MARCH_ETA = 1,  /* eta sweep (+ xi difference)             */
-  CBNS_K_MARCH_ZETA = 2, /* zeta sweep + RK stage combination       */
+  CBNS_K_MARCH_ETA = 1,  /* eta sweep (+ Fv's divergence)           */
+  CBNS_K_MARCH_ZETA = 2, /* zeta + xi sweep (+ Gv's, Ev's divergence) + RK stage combination */
   CBNS_K_VISC_FLUX = 3,  /* viscous fluxes Ev,Fv,Gv at the nodes     */
-  CBNS_K_VISC_DIV = 4,   /* viscous divergence into the stage rhs   */
+  CBNS_K_VISC_DIV = 4,   /* viscous divergence pass (reference-decomposed calls; the fused stage has none) */
   CBNS_K_BC = 5,         /* boundary / ghost fill                   */
   CBNS_K_DT = 6,         /* CFL time step reduction                 */
   CBNS_K_INDICATOR = 7,  /* learned troubled-face flags (hybrid)    */
