@@ -120,6 +120,25 @@ def roofline_of(name, ms, n, p, workload):
             "kernel": name, "avg_ms": ms / n}
 
 
+def step_hbm(kt, steps, ms_step, p, workload):
+    """HBM traffic of the whole step: each kernel's committed rocprofv3
+    FETCH_SIZE + WRITE_SIZE bytes per launch times its launches per step
+    (profiled pass), over the timed median step time (None if not profiled)."""
+    total = 0.0
+    for k, (ms, n) in kt.items():
+        if not n:
+            continue
+        t = measured_traffic(k, p, workload)
+        if t is None:
+            return None
+        total += t * n / steps
+    gbs = total / (ms_step / 1e3) / 1e9
+    return {"bytes_per_step": total, "achieved": gbs, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+            "frac": gbs / PEAK_HBM_GBS,
+            "note": "measured HBM bytes per step (the viscous flux pass at its bytes without "
+                    "the time-step maxima, 6 % fewer than the first stage's) over ms_per_step"}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -341,6 +360,7 @@ def main():
     dom = max(kt, key=lambda k: kt[k][0])
     roof = roofline_of(dom, kt[dom][0], kt[dom][1], p, wl)
     per_kernel = {k: roofline_of(k, ms, n, p, wl) for k, (ms, n) in kt.items() if n}
+    hbm = step_hbm(kt, args.profile_steps, ms_step, p, wl)
 
     forms = None
     if not args.no_forms:
@@ -379,6 +399,7 @@ def main():
                           "timed runs (the timed runs record no events)" % args.profile_steps,
         "kernel_roofline": {k: {"frac": r["frac"], "achieved": r["achieved"], "unit": r["unit"]}
                             for k, r in per_kernel.items()},
+        "step_hbm": hbm,
     }
     print(json.dumps(out), flush=True)
 

@@ -67,3 +67,24 @@ def test_more_than_one_gpu_exits_with_a_message(n):
     assert r.returncode == 2
     assert "one GPU" in r.stderr
     assert r.stdout == ""
+
+
+def test_step_hbm_sums_launches_per_step():
+    # each kernel's committed bytes per launch times its launches per step,
+    # over the step time; a kernel without a committed figure gives None
+    p = params((256, 256, 256), (1, 1, 1))
+    t = {"march_eta": 2e9, "march_zeta": 7e9, "visc_flux": 5.5e9, "bc": 0.3e9, "dt": 1e5}
+    saved = bench.measured_traffic
+    bench.measured_traffic = lambda name, p_, wl: t.get(name)
+    try:
+        kt = {"march_eta": (1.8, 6), "march_zeta": (4.8, 6), "visc_flux": (3.1, 6),
+              "bc": (0.2, 6), "dt": (0.04, 2), "indicator": (0.0, 0)}
+        r = bench.step_hbm(kt, 2, 10.0, p, "tgv")
+        per_step = 3 * (2e9 + 7e9 + 5.5e9 + 0.3e9) + 1e5
+        assert r["bytes_per_step"] == pytest.approx(per_step)
+        assert r["achieved"] == pytest.approx(per_step / 10e-3 / 1e9)
+        assert r["frac"] == pytest.approx(r["achieved"] / bench.PEAK_HBM_GBS)
+        kt["visc_div"] = (0.6, 6)
+        assert bench.step_hbm(kt, 2, 10.0, p, "tgv") is None
+    finally:
+        bench.measured_traffic = saved
