@@ -195,41 +195,18 @@ __global__ void k_bc_zeta(Consts c, Box box, double* __restrict__ Q, double* __r
   }
 }
 
-// periodic wrap of one axis: every storage node whose coordinate along `a`
-// lies in a ghost layer copies Q and the primitives of the wrapped node.
-// Axes are wrapped one after another over the full ghosted extent of the
-// other two, so edges and corners end up right.
-__global__ void k_wrap(Consts c, int a, double* __restrict__ Q, double* __restrict__ prim,
-                       int pmask) {
-  const int b = (a + 1) % 3, cc = (a + 2) % 3;
-  const int P[3] = {c.PX, c.PY, c.PZ};
-  const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long cnt = (long)(2 * G) * P[b] * P[cc];
-  if (t >= cnt) return;
-  const int gl = (int)(t % (2 * G));
-  const int ub = (int)((t / (2 * G)) % P[b]) - G;
-  const int uc = (int)(t / ((long)(2 * G) * P[b])) - G;
-  const int n = axis_n(c, a);
-  const int m = gl < G ? gl - G : n + (gl - G);
-  const int ms = gl < G ? m + n : m - n;
-  int pd[3], ps[3];
-  pd[a] = m; pd[b] = ub; pd[cc] = uc;
-  ps[a] = ms; ps[b] = ub; ps[cc] = uc;
-  const int d = idx(c, pd[0], pd[1], pd[2]);
-  const int s = idx(c, ps[0], ps[1], ps[2]);
-#pragma unroll
-  for (int q = 0; q < NEQ; q++) Q[(long)q * c.S + d] = Q[(long)q * c.S + s];
-#pragma unroll
-  for (int q = 0; q < 6; q++)
-    if (pmask >> q & 1) prim[(long)q * c.S + d] = prim[(long)q * c.S + s];
-}
-
-// All axes periodic: every ghost node copies the node at its coordinates
-// wrapped on all three axes at once (the same values the axis-by-axis wraps
-// produce).  The ghost shell is enumerated slab by slab: zeta slabs (full
-// extent), then eta slabs (zeta in range), then xi slabs (eta, zeta in range).
-__global__ void k_wrap_shell(Consts c, double* __restrict__ Q, double* __restrict__ prim,
-                             int pmask) {
+// Periodic wrap (per: bit a set for each periodic axis a).  Every ghost node
+// that lies in the ghost layers of a periodic axis copies Q and the
+// primitives of the node at its coordinates wrapped on every periodic axis
+// at once.  Those are the values that wrapping one axis after another over
+// the full ghosted extent of the other two leaves at every node, edges and
+// corners included, once the boundary rules of the other axes have written
+// their own ghost layers (they run first); ghost nodes of non-periodic axes
+// alone are left as those rules wrote them.  The ghost shell is enumerated
+// slab by slab: zeta slabs (full extent), then eta slabs (zeta in range),
+// then xi slabs (eta, zeta in range).
+__global__ void k_wrap_shell(Consts c, int per, double* __restrict__ Q,
+                             double* __restrict__ prim, int pmask) {
   const unsigned t = blockIdx.x * blockDim.x + threadIdx.x;
   const unsigned PX = c.PX, PY = c.PY, twoG = 2 * G;
   const unsigned nz = twoG * PX * PY, ny = twoG * PX * (unsigned)c.KM,
@@ -257,9 +234,13 @@ __global__ void k_wrap_shell(Consts c, double* __restrict__ Q, double* __restric
   } else {
     return;
   }
-  const int si = i < 0 ? i + c.IM : (i >= c.IM ? i - c.IM : i);
-  const int sj = j < 0 ? j + c.JM : (j >= c.JM ? j - c.JM : j);
-  const int sk = k < 0 ? k + c.KM : (k >= c.KM ? k - c.KM : k);
+  const bool gi = (per & 1) && (i < 0 || i >= c.IM);
+  const bool gj = (per & 2) && (j < 0 || j >= c.JM);
+  const bool gk = (per & 4) && (k < 0 || k >= c.KM);
+  if (!(gi || gj || gk)) return;  // a ghost node of non-periodic axes only
+  const int si = !(per & 1) ? i : (i < 0 ? i + c.IM : (i >= c.IM ? i - c.IM : i));
+  const int sj = !(per & 2) ? j : (j < 0 ? j + c.JM : (j >= c.JM ? j - c.JM : j));
+  const int sk = !(per & 4) ? k : (k < 0 ? k + c.KM : (k >= c.KM ? k - c.KM : k));
   const int d = idx(c, i, j, k), s = idx(c, si, sj, sk);
 #pragma unroll
   for (int q = 0; q < NEQ; q++) Q[(long)q * c.S + d] = Q[(long)q * c.S + s];
@@ -302,17 +283,13 @@ int launch_bc_only(cbns_block* b, double* Q, int pmask) {
     if ((rc = run(b, box, k_bc_zeta, Q, b->prim))) return rc;
   }
   if (pmask < 0) pmask = c.viscous ? 0x3F : 0x1F;
-  if (periodic(c, 0) && periodic(c, 1) && periodic(c, 2)) {
+  const int per = (periodic(c, 0) ? 1 : 0) | (periodic(c, 1) ? 2 : 0) | (periodic(c, 2) ? 4 : 0);
+  if (per) {
+    // one launch for all periodic axes (one per axis cost 27 us each on a
+    // 256x128x128 block with two periodic axes)
     const long cnt = (long)c.S - (long)c.IM * c.JM * c.KM;
-    k_wrap_shell<<<(unsigned)((cnt + 255) / 256), 256, 0, b->stream>>>(c, Q, b->prim, pmask);
-    CBNS_HIP(b, hipGetLastError());
-    return CBNS_OK;
-  }
-  const int P[3] = {c.PX, c.PY, c.PZ};
-  for (int a = 0; a < 3; a++) {
-    if (!periodic(c, a)) continue;
-    const long cnt = (long)(2 * G) * P[(a + 1) % 3] * P[(a + 2) % 3];
-    k_wrap<<<(unsigned)((cnt + 255) / 256), 256, 0, b->stream>>>(c, a, Q, b->prim, pmask);
+    k_wrap_shell<<<(unsigned)((cnt + 255) / 256), 256, 0, b->stream>>>(c, per, Q, b->prim,
+                                                                       pmask);
     CBNS_HIP(b, hipGetLastError());
   }
   return CBNS_OK;
