@@ -190,7 +190,7 @@ int cbns_create(const cbns_params* p, cbns_block** out) {
     cbns_destroy(b);
     return rc;
   }
-  b->red_n = 4 + 4096;
+  b->red_n = (int)(4 + std::max<long>(4096, time_step_blocks(b->c)));
   if ((rc = dmalloc(b, &b->red, b->red_n))) {
     cbns_destroy(b);
     return rc;
@@ -456,38 +456,32 @@ int cbns_rk3_step(cbns_block* b, double t_end, double* t_cur, double* dt_out) {
   int rc = check_block(b);
   if (rc) return rc;
   if (!t_cur) return fail(b, CBNS_ERR_ARG, "null t_cur");
-  double dt = 0.0;
-  if (b->c.viscous) {
-    // the first stage's viscous flux pass reads every node's primitives and
-    // metrics anyway: it also forms the step's time step on the device (the
-    // same per-node term and maximum as cbns_calc_dt, limited to the rest of
-    // the run), which the stages read there; dt reaches the host once, after
-    // the step.  A NaN step leaves the state untouched and is reported.
-    if ((rc = ensure_stage_scratch(b))) return rc;
-    b->dt_fold = true;
-    b->step_internal = true;
-    b->dt_cap = t_end - *t_cur;
-    for (int s = 1; s <= 3 && !rc; s++) rc = cbns_rk3_stage(b, 0.0, s);
-    b->dt_fold = false;
-    b->step_internal = false;
-    if (rc) return rc;
-    CBNS_HIP(b, hipMemcpyAsync(b->h_scalar + 2, b->red + 2, 2 * sizeof(double),
-                               hipMemcpyDeviceToHost, b->stream));
-    CBNS_HIP(b, hipStreamSynchronize(b->stream));
-    dt = b->h_scalar[2];
-    if (std::isnan(dt)) return fail(b, CBNS_ERR_NAN, "An error occurred while calculating the time step.");
-    *t_cur += dt;
-    if (dt_out) *dt_out = dt;
-    return CBNS_OK;
+  if ((rc = ensure_stage_scratch(b))) return rc;
+  // The step's time step (limited to the rest of the run) is formed on the
+  // device, where the stages read it: on viscous blocks by the first stage's
+  // viscous flux pass, which reads every node's primitives and metrics
+  // anyway, otherwise by the time-step pass of cbns_calc_dt (the same
+  // per-node term and maximum either way).  dt reaches the host once, after
+  // the step.  A NaN step leaves the state untouched and is reported.
+  b->dt_cap = t_end - *t_cur;
+  if (!b->c.viscous) {
+    ProfScope ps(b, CBNS_K_DT);
+    if ((rc = launch_dt_device(b, b->dt_cap, b->red + 2))) return rc;
   }
-  if ((rc = cbns_calc_dt(b, &dt))) return rc;
-  if (t_end - *t_cur < dt) dt = t_end - *t_cur;
+  b->dt_fold = true;
+  b->step_internal = true;
+  for (int s = 1; s <= 3 && !rc; s++) rc = cbns_rk3_stage(b, 0.0, s);
+  b->dt_fold = false;
+  b->step_internal = false;
+  if (rc) return rc;
+  CBNS_HIP(b, hipMemcpyAsync(b->h_scalar + 2, b->red + 2, 2 * sizeof(double),
+                             hipMemcpyDeviceToHost, b->stream));
+  CBNS_HIP(b, hipStreamSynchronize(b->stream));
+  const double dt = b->h_scalar[2];
+  if (std::isnan(dt)) return fail(b, CBNS_ERR_NAN, "An error occurred while calculating the time step.");
   *t_cur += dt;
   if (dt_out) *dt_out = dt;
-  b->step_internal = true;
-  for (int s = 1; s <= 3 && !rc; s++) rc = cbns_rk3_stage(b, dt, s);
-  b->step_internal = false;
-  return rc;
+  return CBNS_OK;
 }
 
 int cbns_set_form(cbns_block* b, int32_t form) {

@@ -360,6 +360,10 @@ int launch_dt_from_partials(cbns_block* b, const double* partial, int n, double 
                             double* out);
 int launch_update_rk3(cbns_block* b, double dt, int stage);
 int launch_calc_dt(cbns_block* b, double* dt_host);
+// workgroups (partial maxima) of the time-step pass, and the pass itself
+// with dt (limited to cap) and the maximum left in out[0..1] on the device
+long time_step_blocks(const Consts& c);
+int launch_dt_device(cbns_block* b, double cap, double* out);
 int launch_sutherland(cbns_block* b);
 int launch_fused_stage(cbns_block* b, double dt, int stage, const double* qin, double* qout);
 bool fused_stage_uses_face_flux();

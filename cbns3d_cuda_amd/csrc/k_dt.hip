@@ -12,26 +12,38 @@ namespace cbns {
 namespace {
 
 constexpr int TPB = 256;
+constexpr int DT_KCH = 32;  // nodes along k per thread of the time-step pass
 
+dim3 time_step_grid(const Consts& c) {
+  return dim3((unsigned)(c.IM + 63) / 64, (unsigned)(c.JM + 3) / 4,
+              (unsigned)(c.KM + DT_KCH - 1) / DT_KCH);
+}
+
+// Per-node spectral-radius sums and their maximum per workgroup: 64 x 4
+// threads on (i, j), each walking DT_KCH nodes along k (node addresses
+// without integer division; loads coalesced along i).
 __global__ void __launch_bounds__(TPB) k_time_step(Consts c, const double* __restrict__ prim,
                                                    const double* __restrict__ met,
                                                    double* __restrict__ partial) {
   __shared__ double lds[TPB / 64];
-  const long N = (long)c.IM * c.JM * c.KM;
+  const int i = (int)blockIdx.x * 64 + (int)threadIdx.x;
+  const int j = (int)blockIdx.y * 4 + (int)threadIdx.y;
+  const int k0 = (int)blockIdx.z * DT_KCH, k1 = min(k0 + DT_KCH, c.KM);
   double lam = -INFINITY;
   int nan_seen = 0;
-  for (long t = (long)blockIdx.x * TPB + threadIdx.x; t < N; t += (long)gridDim.x * TPB) {
-    const int i = (int)(t % c.IM);
-    const int j = (int)((t / c.IM) % c.JM);
-    const int k = (int)(t / ((long)c.IM * c.JM));
-    const double sum = node_lambda(c, prim, met, idx(c, i, j, k));
-    if (isnan(sum)) nan_seen = 1;
-    else lam = fmax(lam, sum);
+  if (i < c.IM && j < c.JM) {
+    for (int k = k0; k < k1; k++) {
+      const double sum = node_lambda(c, prim, met, idx(c, i, j, k));
+      if (isnan(sum)) nan_seen = 1;
+      else lam = fmax(lam, sum);
+    }
   }
-  lam = block_max(lam, lds, threadIdx.x, TPB);
+  const int tid = (int)threadIdx.x + 64 * (int)threadIdx.y;
+  lam = block_max(lam, lds, tid, TPB);
   __syncthreads();
   const int any_nan = __syncthreads_or(nan_seen);
-  if (threadIdx.x == 0) partial[blockIdx.x] = any_nan ? NAN : lam;
+  if (tid == 0)
+    partial[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)] = any_nan ? NAN : lam;
 }
 
 // out[0] = CFL / max, limited to cap (the rest of the run, t_end - t_cur;
@@ -83,15 +95,27 @@ int launch_sutherland(cbns_block* b) {
   return CBNS_OK;
 }
 
-int launch_calc_dt(cbns_block* b, double* dt_host) {
+long time_step_blocks(const Consts& c) {
+  const dim3 g = time_step_grid(c);
+  return (long)g.x * g.y * g.z;
+}
+
+// The time step on the device: out[0] = CFL / max (limited to cap), out[1]
+// = max; NaN if any node's sum is NaN.
+int launch_dt_device(cbns_block* b, double cap, double* out) {
   const Consts& c = b->c;
-  const long N = (long)c.IM * c.JM * c.KM;
-  int nb = (int)((N + TPB - 1) / TPB);
-  if (nb > b->red_n - 4) nb = b->red_n - 4;
-  k_time_step<<<nb, TPB, 0, b->stream>>>(c, b->prim, b->met, b->red + 4);
+  const int nb = (int)time_step_blocks(c);
+  if (nb > b->red_n - 4) return fail(b, CBNS_ERR_STATE, "time-step partials exceed their buffer");
+  k_time_step<<<time_step_grid(c), dim3(64, 4), 0, b->stream>>>(c, b->prim, b->met, b->red + 4);
   CBNS_HIP(b, hipGetLastError());
-  k_dt_final<<<1, TPB, 0, b->stream>>>(b->red + 4, nb, c.CFL, INFINITY, b->red);
+  k_dt_final<<<1, TPB, 0, b->stream>>>(b->red + 4, nb, c.CFL, cap, out);
   CBNS_HIP(b, hipGetLastError());
+  return CBNS_OK;
+}
+
+int launch_calc_dt(cbns_block* b, double* dt_host) {
+  int rc = launch_dt_device(b, INFINITY, b->red);
+  if (rc) return rc;
   CBNS_HIP(b, hipMemcpyAsync(b->h_scalar, b->red, 2 * sizeof(double), hipMemcpyDeviceToHost,
                              b->stream));
   CBNS_HIP(b, hipStreamSynchronize(b->stream));

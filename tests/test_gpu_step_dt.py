@@ -1,12 +1,12 @@
-"""The time step of cbns_rk3_step on viscous blocks, and the primitives a
-step leaves behind.
+"""The time step of cbns_rk3_step, and the primitives a step leaves behind.
 
-There the first stage's viscous flux pass forms the step's dt on the device
-(the per-node spectral-radius sum of src/time_step_kernel.cu:43-94 and its
-maximum), instead of a separate cbns_calc_dt pass.  It must equal
-cbns_calc_dt bit for bit, including the limit to the rest of the run
-(src/Block3d.cpp:241-246), and a NaN time step must be reported with the
-state left as it was (the reference throws before the update,
+The step forms its dt on the device, where the stages read it: on viscous
+blocks the first stage's viscous flux pass (the per-node spectral-radius sum
+of src/time_step_kernel.cu:43-94 and its maximum), on inviscid blocks the
+time-step pass of cbns_calc_dt, with no host round trip before the stages.
+It must equal cbns_calc_dt bit for bit, including the limit to the rest of
+the run (src/Block3d.cpp:241-246), and a NaN time step must be reported with
+the state left as it was (the reference throws before the update,
 src/calc_dt.cu:56-58).
 """
 import numpy as np
@@ -22,6 +22,10 @@ CASES = {
     "cgrid_viscous": lambda: cases.c_grid_channel(viscous=1),
     "box_70x9x8_rpp_visc": lambda: cases.smooth_box(
         70, 9, 8, viscous=1, bc=(cases.BC_REF, cases.BC_PERIODIC, cases.BC_PERIODIC)),
+    "vortex": lambda: cases.isentropic_vortex(16),
+    "cgrid_inviscid": lambda: cases.c_grid_channel(viscous=0),
+    "box_70x9x8_rpp": lambda: cases.smooth_box(
+        70, 9, 8, viscous=0, bc=(cases.BC_REF, cases.BC_PERIODIC, cases.BC_PERIODIC)),
 }
 
 
@@ -58,8 +62,9 @@ def test_step_dt_equals_calc_dt(name, form):
         b.free()
 
 
-def test_nan_step_reported_and_state_kept():
-    case = cases.taylor_green(16)
+@pytest.mark.parametrize("name", ["tgv", "vortex", "box_70x9x8_rpp"])
+def test_nan_step_reported_and_state_kept(name):
+    case = CASES[name]()
     print("case", case.name, flush=True)
     b = _block(case)
     try:
