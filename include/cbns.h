@@ -143,7 +143,9 @@ int cbns_update_rk3(cbns_block* blk, double dt, int32_t stage);
  * src/Block3d.cpp:249-287, run as the MI355X sweep kernels. */
 int cbns_rk3_stage(cbns_block* blk, double dt, int32_t stage);
 /* One pass of the time loop src/Block3d.cpp:239-287: dt (clamped to t_end),
- * t_cur += dt, three fused stages. */
+ * t_cur += dt, three fused stages.  dt is formed on the device (bit-equal to
+ * cbns_calc_dt) and read there by the stages; a NaN dt returns
+ * CBNS_ERR_NAN with the state unchanged. */
 int cbns_rk3_step(cbns_block* blk, double t_end, double* t_cur, double* dt_out);
 /* arithmetic form of cbns_rk3_stage / cbns_rk3_step (CBNS_FORM_*) */
 int cbns_set_form(cbns_block* blk, int32_t form);
